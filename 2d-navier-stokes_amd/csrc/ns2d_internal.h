@@ -1,0 +1,106 @@
+// Internal definitions shared by the kernels and the C-ABI host code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace ns2d {
+
+// RK4 Butcher tableau and AB4 weights (reference Solver/src/solver.c:29-47)
+constexpr double RK4_A21 = 0.5, RK4_A32 = 0.5, RK4_A43 = 1.0;
+constexpr double RK4_B1 = 1.0 / 6.0, RK4_B2 = 1.0 / 3.0, RK4_B3 = 1.0 / 3.0, RK4_B4 = 1.0 / 6.0;
+constexpr double AB4_1 = 55.0 / 24.0, AB4_2 = -59.0 / 24.0, AB4_3 = 37.0 / 24.0, AB4_4 = -3.0 / 8.0;
+// data_types.h:171-173
+constexpr double MAX_VORT_LIM = 1e100;
+
+enum { FILTER_23 = 0, FILTER_HOULI_REF = 1, FILTER_HOULI = 2, FILTER_NONE = 3 };
+enum { VISC_K2 = 0, VISC_HYPER = 1, VISC_K2_DRAG = 2, VISC_HYPER_DRAG = 3 };
+
+// kpass mode bits
+enum { KP_FWD = 1, KP_INV = 2, KP_CONJFIX = 4, KP_STAT = 8, KP_ITER = 16, KP_RAW = 32 };
+// kpass epilogues
+enum { EPI_FROM_STATE = 0, EPI_FROM_INPUT = 1, EPI_RHS = 2, EPI_RK1 = 3, EPI_RK2 = 4, EPI_RK3 = 5,
+       EPI_RK4CN = 6, EPI_AB4 = 7 };
+// rpass mode bits
+enum { RP_NL = 1, RP_W = 2 };
+
+// device status words (unsigned 64-bit)
+enum { STAT_WMAX_CUR = 0, STAT_WMAX_LAST = 1, STAT_FIRST_BAD = 2, STAT_ITER = 3, STAT_N = 4 };
+
+// per-bin reduction quantities of the measures kernel
+enum {
+  Q_ENRG = 0, Q_ENST, Q_PALIN, Q_ENRG_DISS, Q_ENST_DISS, Q_DIV,
+  Q_DERIV, Q_DISS, Q_DERIV_K2, Q_DISS_K2,
+  Q_SB_DERIV, Q_SB_DISS, Q_SB_DERIV_K2, Q_SB_DISS_K2,
+  NQ
+};
+
+struct PhysArgs {
+  int nx, ny;
+  double nu, hyper_pow, alpha, drag_pow;
+  int filter;
+  double kmax_sqr;      // (int)(Nx/3)^2, solver.c:1048
+  int kolm_kill;        // 2/3 filter also zeroes kx=0 (KOLM, solver.c:1072)
+  double houli_edge;    // Hou-Li (integer-division form) value on the Nyquist row/column
+  double houli_corner;  // ... and on the corner
+  double norm2;         // (1/(Nx*Ny))^2, solver.c:1005
+  int force_on, force_k;
+  double force_val;
+};
+
+struct KPassArgs {
+  double2* w_hat;
+  double2* acc;
+  double2* h_old;
+  double2* h_mid;
+  double2* h_new;
+  double2* hist_store;
+  const double2* nin;
+  const double2* sin;
+  double2* g[4];
+  double2* kout;
+  const double2* conj_src;
+  double2* conj_dst;
+  unsigned long long* stat;
+  const double2* tw;
+  int nyf, ld;
+  int mode, epi, visc_kind;
+  double dt, coef;
+  PhysArgs ph;
+};
+
+struct RPassArgs {
+  const double2* g[4];
+  double2* nout;
+  double* wreal;
+  unsigned long long* wmax;
+  const double2* tw;
+  int ld, mode;
+  double wscale;
+};
+
+struct MeasArgs {
+  const double2* w_hat;
+  const double2* rhs;
+  double* out;
+  int nx, ny, ld, visc_kind, quirk_enst_diss;
+  double sbst_lo, sbst_hi;
+  PhysArgs ph;
+};
+
+// blockIdx -> work item map that keeps consecutive work items on one XCD
+// (blocks are dealt round-robin over the 8 XCDs, each with its own L2)
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  constexpr int NXCD = 8;
+  if (n < NXCD) return b;
+  const int q = n / NXCD, r = n % NXCD;
+  const int xcd = b % NXCD, j = b / NXCD;
+  return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + j;
+}
+
+hipError_t launch_kpass(int nx, const KPassArgs& a, hipStream_t s);
+hipError_t launch_rpass(int nx, int ny, const RPassArgs& a, hipStream_t s);
+hipError_t launch_r2c_rows(int nx, int ny, const double* in, double2* out, const double2* tw, int ld,
+                           hipStream_t s);
+hipError_t launch_measures(int nbins, const MeasArgs& a, hipStream_t s);
+
+}  // namespace ns2d

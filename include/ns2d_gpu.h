@@ -1,0 +1,157 @@
+/*
+ * ns2d_gpu.h -- C ABI of the MI355X (gfx950) pseudospectral 2-D Navier-Stokes
+ * vorticity solver: the right-hand side and the RK4-CN / AB4 time integrators
+ * of EndCar808/2D-Navier-Stokes, plus its system measures and max-vorticity
+ * check, computed in FP64 on one GPU.
+ *
+ * Every entry point replaces a function of the reference solver
+ * (Solver/src/...); the reference exchanges fftw_complex arrays in the layout
+ * w_hat[Nx][Ny/2+1] (x slow, interleaved re/im, unnormalised FFTW units:
+ * w_hat = r2c(w)).  The host buffers taken and returned here use exactly that
+ * layout, so a caller can pass run_data->w_hat directly.
+ *
+ * Conventions: all functions return NS2D_OK (0) or an NS2D_ERR_* code and set a
+ * thread-local message readable with ns2d_last_error().  The caller owns host
+ * buffers; a handle owns its device memory and one HIP stream (default
+ * priority).  A handle is not re-entrant: use one handle per host thread.
+ */
+#ifndef NS2D_GPU_H
+#define NS2D_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NS2D_OK 0
+#define NS2D_ERR_INVALID 1     /* bad argument / unsupported size            */
+#define NS2D_ERR_HIP 2         /* HIP runtime failure                         */
+#define NS2D_ERR_BLOWUP 3      /* max|w| >= 1e100 (reference SystemCheck)     */
+#define NS2D_ERR_STATE 4       /* call not valid in the handle's state        */
+
+/* dealiasing filter (reference ApplyDealiasing, Solver/src/solver.c:1039-1099) */
+#define NS2D_FILTER_23 0         /* 2/3 isotropic cutoff (reference default)   */
+#define NS2D_FILTER_HOULI_REF 1  /* Hou-Li with the reference's integer division */
+#define NS2D_FILTER_HOULI 2      /* Hou-Li exp(-36 (|k|/kmax)^36)               */
+#define NS2D_FILTER_NONE 3
+
+/* forcing (reference force.c) */
+#define NS2D_FORCING_NONE 0
+#define NS2D_FORCING_KOLM 1      /* f_hat(0, k_f) = -0.5 F k_f, force.c:535-538 */
+
+/* integrators (reference RK4Step / AB4Step) */
+#define NS2D_INT_RK4CN 0         /* RK4 + Crank-Nicolson viscous term, solver.c:625-873 */
+#define NS2D_INT_AB4 1           /* CN-AB4 with 3 RK4CN start-up steps, solver.c:261-350 */
+
+/* reference quirks (bit set = reproduce the reference's behaviour) */
+#define NS2D_QUIRK_CONJ_GLOBAL 0x01    /* ForceConjugacy writes the global w_hat, solver.c:2089-2101 */
+#define NS2D_QUIRK_VISC_ASSIGN 0x02    /* HYPER_VISC_FLAG = HYPER_VISC assignment, solver.c:843   */
+#define NS2D_QUIRK_KOLM_UNSCALED 0x04  /* forcing added without the N^2 scale, force.c:537       */
+#define NS2D_QUIRK_KOLM_KX0_KILL 0x08  /* 2/3 filter zeroes kx=0 under KOLM, solver.c:1072-1078   */
+#define NS2D_QUIRK_ENST_DISS 0x10      /* enst_diss drops pre_fac off the edge, sys_msr.c:225     */
+#define NS2D_QUIRKS_REF 0x1f
+
+typedef struct ns2d_params {
+  int nx, ny;            /* -n Nx -n Ny (powers of two, 8..4096)               */
+  double nu;             /* -v nu                                              */
+  int hyper_flag;        /* -v (2nd): HYPER_VISC_FLAG                          */
+  double hyper_pow;      /* -v (3rd): HYPER_VISC_POW                           */
+  double alpha;          /* -d alpha: EKMN_ALPHA                               */
+  int drag_flag;         /* -d (2nd): EKMN_DRAG_FLAG                           */
+  double drag_pow;       /* -d (3rd): EKMN_DRAG_POW                            */
+  int filter;            /* NS2D_FILTER_*                                      */
+  int forcing;           /* NS2D_FORCING_*                                     */
+  int force_k;           /* -f (2nd): forced wavenumber k_f                    */
+  double force_scale;    /* -f (1st): forcing amplitude F                      */
+  int integrator;        /* NS2D_INT_*                                         */
+  unsigned int quirks;   /* NS2D_QUIRK_* bits                                  */
+  int check_every;       /* 1: per-step max|w| blow-up check like SystemCheck  */
+  int device;            /* HIP device ordinal                                 */
+} ns2d_params;
+
+/* Totals of ComputeSystemMeasurables (sys_msr.c:32-373), already normalised
+ * as the reference normalises them (sys_msr.c:340-372). */
+typedef struct ns2d_measures {
+  double energy, enstrophy, palinstrophy, divergence;
+  double enrg_diss, enst_diss, forcing;
+  /* flux subsets (LWR_SBST_LIM <= |k| < UPR_SBST_LIM), filled when requested */
+  double d_enrg_dt_sbst, enrg_flux_sbst, enrg_diss_sbst;
+  double d_enst_dt_sbst, enst_flux_sbst, enst_diss_sbst;
+} ns2d_measures_t;
+
+typedef struct ns2d_info {
+  long iters;              /* completed steps since the last ns2d_set_state       */
+  long first_blowup_iter;  /* step after which max|w| >= 1e100 was seen, or -1    */
+  double last_max_vort;    /* max|w| of the last checked state                    */
+  int hyper_flag;          /* HYPER_VISC_FLAG as the reference would now hold it  */
+  int n_spect;             /* spectrum length, sys_msr.c:394                      */
+  int ld;                  /* device row stride (complex elements)                */
+} ns2d_info_t;
+
+typedef struct ns2d_handle ns2d_t;
+
+/* Fill *p with the reference's CLI defaults (utils.c:51-85) for an nx x ny grid. */
+int ns2d_params_default(ns2d_params* p, int nx, int ny);
+
+/* Replaces AllocateMemory + InitializeFFTWPlans + InitializeSpaceVariables +
+ * InitializeForcing (solver.c:2603-2901, 2022-2068, force.c:28-102). */
+int ns2d_create(const ns2d_params* p, ns2d_t** out);
+int ns2d_destroy(ns2d_t* h);
+
+/* Load / read the state run_data->w_hat (host memory, [Nx][Ny/2+1] complex).
+ * set_state resets the step counter, AB4 history and blow-up record. */
+int ns2d_set_state(ns2d_t* h, const double* w_hat);
+int ns2d_get_state(ns2d_t* h, double* w_hat);
+/* Same, with a device pointer to a [Nx][Ny/2+1] complex array. */
+int ns2d_set_state_device(ns2d_t* h, const void* d_w_hat);
+int ns2d_get_state_device(ns2d_t* h, void* d_w_hat);
+
+/* Advance nsteps steps of size dt with the configured integrator: replaces
+ * the RK4Step / AB4Step calls of the SpectralSolve loop (solver.c:142-236)
+ * together with the per-step SystemCheck (solver.c:2301-2326) when
+ * check_every is set.  Returns NS2D_ERR_BLOWUP if max|w| >= 1e100 was seen
+ * (ns2d_info_t.first_blowup_iter tells after which step). */
+int ns2d_step(ns2d_t* h, double dt, long nsteps);
+
+/* NonlinearRHSBatch (solver.c:884-1032): out = dealiased, forced nonlinear
+ * term of `in`, both host [Nx][Ny/2+1] complex.  As in the reference, with
+ * NS2D_QUIRK_CONJ_GLOBAL the handle's state gets w_hat[kx<0,0] = conj(in[|kx|,0]). */
+int ns2d_rhs(ns2d_t* h, const double* in, double* out);
+
+/* ComputeSystemMeasurables (sys_msr.c:32-373) on the current state.
+ * enrg_spec / enst_spec: n_spect doubles each (or NULL).  flux_spec: NULL or
+ * 6*n_spect doubles in the order d_enst_dt, enst_diss, enst_flux, d_enrg_dt,
+ * enrg_diss, enrg_flux (hdf5_funcs.c:323-336).  with_flux != 0 evaluates the
+ * extra nonlinear term the flux terms need (sys_msr.c:125-133). */
+int ns2d_measures(ns2d_t* h, int with_flux, ns2d_measures_t* m, double* enrg_spec, double* enst_spec,
+                  double* flux_spec);
+
+/* GetMaxData("VORT") (solver.c:2171-2295): max |c2r(w_hat)| / (Nx*Ny). */
+int ns2d_max_vort(ns2d_t* h, double* out);
+
+/* The real-space vorticity written as "w" (hdf5_funcs.c:174,796):
+ * w = c2r(w_hat)/(Nx*Ny), host [Nx][Ny] doubles. */
+int ns2d_real_vort(ns2d_t* h, double* w);
+
+/* Unnormalised forward transform of a real field, as the ICs use
+ * (fftw_mpi_execute_dft_r2c, solver.c:1178): w [Nx][Ny] -> w_hat [Nx][Ny/2+1]. */
+int ns2d_r2c(ns2d_t* h, const double* w, double* w_hat);
+
+int ns2d_get_info(ns2d_t* h, ns2d_info_t* info);
+int ns2d_sync(ns2d_t* h);
+/* The handle's hipStream_t, for callers that order their own work after it. */
+int ns2d_get_stream(ns2d_t* h, void** stream);
+
+/* Run nsteps steps with HIP events around every kernel launch and report the
+ * mean duration (ms) of the column pass (kpass) and the row pass (rpass) and
+ * the total elapsed time (ms).  The state advances as with ns2d_step. */
+int ns2d_profile_steps(ns2d_t* h, double dt, long nsteps, double* kpass_ms, double* rpass_ms,
+                       double* total_ms);
+
+const char* ns2d_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NS2D_GPU_H */

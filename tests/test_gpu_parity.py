@@ -1,0 +1,153 @@
+"""GPU parity: the HIP path, called through the C ABI, against the oracle.
+
+Tolerances (FP64): single RHS evaluations and transforms agree to 1e-12
+relative L2; trajectories to 1e-10 relative L2 of w_hat (the bar for the
+Taylor-Green 1000-step run); energy / enstrophy series to 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+import ns2d_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def solver(p: O.Params, check=1):
+    import ns2d
+    cfg = ns2d.Config(Nx=p.Nx, Ny=p.Ny, nu=p.nu, hyper_flag=p.hyper_flag, hyper_pow=p.hyper_pow,
+                      alpha=p.alpha, drag_flag=p.drag_flag, drag_pow=p.drag_pow, filter=p.filter,
+                      forcing=p.forcing, force_k=p.force_k, force_scale=p.force_scale,
+                      integrator=p.integrator, quirks=p.quirks, check_every=check)
+    return ns2d.Solver(cfg)
+
+
+SIZES = [(8, 8), (16, 16), (32, 64), (64, 32), (128, 128), (256, 256), (512, 512), (1024, 1024)]
+
+
+@pytest.mark.parametrize("Nx,Ny", SIZES)
+def test_r2c_matches_numpy(Nx, Ny):
+    print(f"r2c {Nx}x{Ny}", flush=True)
+    p = O.Params(Nx, Ny)
+    rng = np.random.default_rng(Nx * 7 + Ny)
+    w = rng.standard_normal((Nx, Ny))
+    with solver(p) as s:
+        got = s.r2c(w)
+    assert rel(got, O.r2c(w)) < 1e-13
+
+
+@pytest.mark.parametrize("Nx,Ny", SIZES)
+@pytest.mark.parametrize("filt", ["23", "houli_ref"])
+def test_rhs_matches_oracle(Nx, Ny, filt):
+    print(f"rhs {Nx}x{Ny} {filt}", flush=True)
+    p = O.Params(Nx, Ny, nu=1e-3, filter=filt)
+    w = O.ic_random_spectrum(p, seed=Nx + 3 * Ny, k0=max(2.0, Nx / 16))
+    # make the ky=0 column non-Hermitian to exercise the c2r projection
+    w[Nx - 1, 0] += 0.3 * (1 + 1j) * np.abs(w).max()
+    K_ref = O.nonlinear_rhs(w, p)
+    st = w.copy()
+    O.force_conjugacy(w, st)
+    with solver(p) as s:
+        s.set_state(w)
+        K = s.nonlinear_rhs(w)
+        st_gpu = s.get_state()
+    assert rel(K, K_ref) < 1e-12
+    assert rel(st_gpu, st) < 1e-15
+
+
+@pytest.mark.parametrize("Nx,Ny", [(16, 16), (64, 64), (128, 128), (256, 128)])
+def test_rhs_kolmogorov_forcing(Nx, Ny):
+    print(f"rhs kolm {Nx}x{Ny}", flush=True)
+    p = O.Params(Nx, Ny, nu=1e-3, forcing="KOLM", force_k=4, force_scale=1.0)
+    w = O.ic_random_spectrum(p, seed=5)
+    with solver(p) as s:
+        K = s.nonlinear_rhs(w)
+    assert rel(K, O.nonlinear_rhs(w, p)) < 1e-12
+
+
+@pytest.mark.parametrize("Nx,Ny", [(16, 16), (64, 64), (128, 128), (512, 256)])
+def test_max_and_real_vorticity(Nx, Ny):
+    print(f"maxvort {Nx}x{Ny}", flush=True)
+    p = O.Params(Nx, Ny)
+    w = O.ic_random_spectrum(p, seed=11)
+    w[3, 0] += 0.1 * np.abs(w).max()
+    with solver(p) as s:
+        s.set_state(w)
+        m = s.max_vort()
+        wr = s.real_vorticity()
+    ref = O.real_vorticity(w, p)
+    assert rel(wr, ref) < 1e-13
+    assert abs(m - O.max_vort(w, p)) <= 1e-13 * abs(m)
+
+
+@pytest.mark.parametrize("integ", ["RK4CN", "AB4"])
+@pytest.mark.parametrize("Nx,Ny,filt", [(32, 32, "23"), (64, 64, "houli_ref"), (128, 64, "23")])
+def test_steps_match_oracle(Nx, Ny, filt, integ):
+    print(f"steps {Nx}x{Ny} {filt} {integ}", flush=True)
+    p = O.Params(Nx, Ny, nu=2e-3, hyper_flag=0, hyper_pow=1.0, filter=filt, integrator=integ)
+    w0 = O.ic_random_spectrum(p, seed=21, k0=4.0)
+    st = O.State(w0, p)
+    O.step(st, 1e-3, 12)
+    with solver(p) as s:
+        s.set_state(w0)
+        s.spectral_step(1e-3, 5)
+        s.spectral_step(1e-3, 7)
+        got = s.get_state()
+        info = s.info()
+    assert info["iters"] == 12
+    assert info["hyper_flag"] == st.hyper_flag
+    assert rel(got, st.w_hat) < 1e-12
+
+
+def test_taylor_green_1000_steps():
+    """Config 1: TG 128^2, nu=1e-3 (-v 1e-3 -v 0 -v 1.0), 2/3, RK4CN, dt=1e-3, 1000 steps."""
+    print("tg 128 x1000", flush=True)
+    p = O.Params(128, 128, nu=1e-3, hyper_flag=0, hyper_pow=1.0)
+    w0 = O.ic_taylor_green(p)
+    st = O.State(w0, p)
+    O.step(st, 1e-3, 1000)
+    with solver(p) as s:
+        s.set_state(w0)
+        s.spectral_step(1e-3, 1000)
+        got = s.get_state()
+        wr = s.real_vorticity()
+    assert rel(got, st.w_hat) < 1e-10
+    x, y = O.collocation(128, 128)
+    exact = 2.0 * np.cos(x) * np.cos(y) * np.exp(-2.0 * 1e-3 * 1.0)
+    assert np.abs(wr - exact).max() < 1e-12
+
+
+@pytest.mark.parametrize("with_flux", [False, True])
+@pytest.mark.parametrize("Nx,Ny,forcing", [(64, 64, "NONE"), (128, 128, "KOLM"), (256, 128, "NONE")])
+def test_measures_match_oracle(Nx, Ny, forcing, with_flux):
+    print(f"measures {Nx}x{Ny} {forcing} flux={with_flux}", flush=True)
+    p = O.Params(Nx, Ny, nu=1e-3, hyper_flag=1, hyper_pow=2.0, alpha=0.1, drag_flag=1,
+                 drag_pow=-2.0, forcing=forcing, force_k=3, force_scale=1.0)
+    w = O.ic_random_spectrum(p, seed=2)
+    w[Nx - 2, 0] += 0.05 * np.abs(w).max()     # non-Hermitian ky=0 entry (measures see it)
+    wref = w.copy()
+    rhs = None
+    if with_flux:
+        rhs = O.nonlinear_rhs(wref, p)
+        O.force_conjugacy(wref, wref)
+        if forcing == "KOLM":
+            rhs[0, p.force_k] -= O.forcing_value(p)
+    ref = O.measures(wref, p, rhs=rhs)
+    with solver(p) as s:
+        s.set_state(w)
+        got = s.system_measures(with_flux=with_flux)
+    for k in ("energy", "enstrophy", "palinstrophy", "enrg_diss", "enst_diss", "forcing"):
+        assert abs(got[k] - ref[k]) <= 1e-12 * max(abs(ref[k]), 1e-300), k
+    assert got["divergence"] < 1e-20
+    assert rel(got["enrg_spect"], ref["enrg_spect"]) < 1e-12
+    assert rel(got["enst_spect"], ref["enst_spect"]) < 1e-12
+    if with_flux:
+        for k in ("enrg_flux_sbst", "enrg_diss_sbst", "enst_flux_sbst", "enst_diss_sbst",
+                  "d_enrg_dt_sbst", "d_enst_dt_sbst"):
+            assert abs(got[k] - ref[k]) <= 1e-10 * max(abs(ref[k]), 1e-30) + 1e-14, k
+        for k in ("d_enst_dt_spect", "enst_diss_spect", "enst_flux_spect",
+                  "d_enrg_dt_spect", "enrg_diss_spect", "enrg_flux_spect"):
+            assert rel(got[k], ref[k]) < 1e-10, k
