@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -50,8 +51,15 @@ struct ns2d_handle {
   int nyf = 0, ld = 0, nspect = 0;
   size_t M = 0;  // nx * ld complex elements per spectral array
   hipStream_t stream = nullptr;
-  double2 *w_hat = nullptr, *acc = nullptr, *g[4] = {nullptr, nullptr, nullptr, nullptr};
-  double2 *nbuf = nullptr, *kout = nullptr, *sbuf = nullptr, *hist[3] = {nullptr, nullptr, nullptr};
+  bool own_stream = true;
+  double2 *w[2] = {nullptr, nullptr};  // state ping-pong: w[wcur] is run_data->w_hat
+  int wcur = 0;
+  double2 *acc = nullptr, *g[4] = {nullptr, nullptr, nullptr, nullptr};
+  double2 *nbuf = nullptr, *kout = nullptr, *sbuf = nullptr, *hist[4] = {nullptr, nullptr, nullptr, nullptr};
+  int kvar = 0;            // kpass variant (tuning)
+  int rvar = 0;            // rpass variant (tuning)
+  double* lam_tab[4] = {nullptr, nullptr, nullptr, nullptr};  // per VISC_* kind
+  double* filt_tab = nullptr;
   double2 *conj[2] = {nullptr, nullptr}, *tw = nullptr;
   unsigned long long* stat = nullptr;
   double* meas = nullptr;
@@ -62,6 +70,15 @@ struct ns2d_handle {
   long iters = 0;
   int hyper_eff = 0;       // HYPER_VISC_FLAG as mutated by the reference (Q3)
   PhysArgs ph{};
+  // hipGraph of one period of steps (kernel launches only; see step_impl)
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  struct {
+    double dt;
+    int wcur, cur, ring, kvar, rvar, check, period;
+    const double* lam;
+  } gkey{};
   // profiling
   bool profiling = false;
   std::vector<hipEvent_t> ev;
@@ -86,13 +103,47 @@ int visc_kind_plain(const ns2d_handle* h) {
   return VISC_K2;
 }
 
+// Per-mode viscous / drag coefficient of the CN factor for one VISC_* kind,
+// evaluated with the host libm pow exactly as the reference writes it
+// (solver.c:835-850, sys_msr.c:160-175); built on first use.
+const double* lam_table(ns2d_handle* h, int kind) {
+  if (h->lam_tab[kind]) return h->lam_tab[kind];
+  const int nx = h->p.nx, ld = h->ld;
+  std::vector<double> t((size_t)nx * ld, 0.0);
+  for (int i = 0; i < nx; ++i) {
+    const long kx = i <= nx / 2 ? i : i - nx;
+    for (int j = 0; j < h->nyf; ++j) {
+      if (i == 0 && j == 0) continue;
+      const double k_sqr = (double)(kx * kx + (long)j * j);
+      const ns2d_params& p = h->p;
+      double v;
+      switch (kind) {
+        case VISC_HYPER_DRAG: v = p.nu * std::pow(k_sqr, p.hyper_pow) + p.alpha * std::pow(k_sqr, p.drag_pow); break;
+        case VISC_K2_DRAG: v = p.nu * k_sqr + p.alpha * std::pow(k_sqr, p.drag_pow); break;
+        case VISC_HYPER: v = p.nu * std::pow(k_sqr, p.hyper_pow); break;
+        default: v = p.nu * k_sqr; break;
+      }
+      t[(size_t)i * ld + j] = v;
+    }
+  }
+  double* d = nullptr;
+  if (hipMalloc((void**)&d, sizeof(double) * t.size()) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  h->lam_tab[kind] = d;
+  return d;
+}
+
 void after_rk_update(ns2d_handle* h) {
   if ((h->p.quirks & NS2D_QUIRK_VISC_ASSIGN) && h->p.drag_flag != 1) h->hyper_eff = 1;
 }
 
 KPassArgs kbase(ns2d_handle* h) {
   KPassArgs a{};
-  a.w_hat = h->w_hat;
+  a.w_in = h->w[h->wcur];
+  a.w_out = h->w[h->wcur];
   a.acc = h->acc;
   a.nin = h->nbuf;
   for (int f = 0; f < 4; ++f) a.g[f] = h->g[f];
@@ -113,6 +164,7 @@ RPassArgs rbase(ns2d_handle* h, int mode) {
   a.wmax = h->stat + STAT_WMAX_CUR;
   a.tw = h->tw;
   a.ld = h->ld;
+  a.nx = h->p.nx;
   a.mode = mode;
   a.wscale = 1.0 / (double)((long)h->p.nx * h->p.ny);
   return a;
@@ -139,7 +191,7 @@ void ev_end(ns2d_handle* h) {
 
 int run_kpass(ns2d_handle* h, const KPassArgs& a) {
   ev_begin(h, KIND_K);
-  hipError_t e = launch_kpass(h->p.nx, a, h->stream);
+  hipError_t e = launch_kpass(h->p.nx, h->kvar, a, h->stream);
   ev_end(h);
   if (e != hipSuccess) return fail(NS2D_ERR_HIP, "kpass launch: %s", hipGetErrorString(e));
   return NS2D_OK;
@@ -147,7 +199,7 @@ int run_kpass(ns2d_handle* h, const KPassArgs& a) {
 
 int run_rpass(ns2d_handle* h, const RPassArgs& a) {
   ev_begin(h, KIND_R);
-  hipError_t e = launch_rpass(h->p.nx, h->p.ny, a, h->stream);
+  hipError_t e = launch_rpass(h->p.nx, h->p.ny, h->rvar, a, h->stream);
   ev_end(h);
   if (e != hipSuccess) return fail(NS2D_ERR_HIP, "rpass launch: %s", hipGetErrorString(e));
   return NS2D_OK;
@@ -168,54 +220,74 @@ int prepare_inv(ns2d_handle* h) {
 
 int conj_mode(const ns2d_handle* h) { return (h->p.quirks & NS2D_QUIRK_CONJ_GLOBAL) ? KP_CONJFIX : 0; }
 
-int rk4cn_step(ns2d_handle* h, double dt, bool check) {
-  const long it = h->iters + 1;
-  double2* hist_store = nullptr;
-  if (h->p.integrator == NS2D_INT_AB4 && it <= 3) hist_store = h->hist[it - 1];  // solver.c:283
-  const int epis[4] = {EPI_RK1, EPI_RK2, EPI_RK3, EPI_RK4CN};
+// Arguments of stage s (0..3) of an RK4CN step (RK4Step, solver.c:625-873)
+// from the handle's current buffers.  The ky=0 column hand-off alternates
+// conj[] every stage; the new state goes to the other state buffer.
+int rk4cn_stage_args(ns2d_handle* h, double dt, bool check, int s, RPassArgs* ra, KPassArgs* ka) {
+  static const int epis[4] = {EPI_RK1, EPI_RK2, EPI_RK3, EPI_RK4CN};
   const double coefs[4] = {dt * RK4_A21, dt * RK4_A32, dt * RK4_A43, 0.0};
+  const long it = h->iters + 1;
+  *ra = rbase(h, RP_NL | ((check && s == 0) ? RP_W : 0));
+  KPassArgs a = kbase(h);
+  a.mode = KP_FWD | KP_INV | conj_mode(h);
+  if (check && s == 0) a.mode |= KP_STAT;
+  if (s == 3) a.mode |= KP_ITER;
+  a.epi = epis[s];
+  a.coef = coefs[s];
+  a.dt = dt;
+  a.lam = lam_table(h, visc_kind_rk(h));
+  if (!a.lam) return fail(NS2D_ERR_HIP, "coefficient table allocation failed");
+  // AB4 start-up keeps K1 of its RK4CN steps (solver.c:283)
+  a.hist_store = (s == 0 && h->p.integrator == NS2D_INT_AB4 && it <= 3) ? h->hist[it - 1] : nullptr;
+  a.conj_src = h->conj[h->cur ^ (s & 1)];
+  a.conj_dst = h->conj[h->cur ^ ((s + 1) & 1)];
+  if (s == 3) a.w_out = h->w[h->wcur ^ 1];
+  *ka = a;
+  return NS2D_OK;
+}
+
+int rk4cn_step(ns2d_handle* h, double dt, bool check) {
   for (int s = 0; s < 4; ++s) {
-    int rc = run_rpass(h, rbase(h, RP_NL | ((check && s == 0) ? RP_W : 0)));
+    RPassArgs ra;
+    KPassArgs ka;
+    int rc = rk4cn_stage_args(h, dt, check, s, &ra, &ka);
     if (rc) return rc;
-    KPassArgs a = kbase(h);
-    a.mode = KP_FWD | KP_INV | conj_mode(h);
-    if (check && s == 0) a.mode |= KP_STAT;
-    if (s == 3) a.mode |= KP_ITER;
-    a.epi = epis[s];
-    a.coef = coefs[s];
-    a.dt = dt;
-    a.visc_kind = visc_kind_rk(h);
-    a.hist_store = (s == 0) ? hist_store : nullptr;
-    a.conj_src = h->conj[h->cur];
-    a.conj_dst = h->conj[h->cur ^ 1];
-    rc = run_kpass(h, a);
+    rc = run_rpass(h, ra);
     if (rc) return rc;
-    h->cur ^= 1;
+    rc = run_kpass(h, ka);
+    if (rc) return rc;
   }
   after_rk_update(h);
-  h->iters = it;
+  h->iters += 1;
+  h->wcur ^= 1;
   return NS2D_OK;
 }
 
 int ab4_step(ns2d_handle* h, double dt, bool check) {
   const long it = h->iters + 1;
   if (it <= 3) return rk4cn_step(h, dt, check);
-  const long m = it - 4;  // AB iteration: slots hold N(n-3), N(n-2), N(n-1)
+  // AB iteration m: slots m%4, (m+1)%4, (m+2)%4 hold N(n-3), N(n-2), N(n-1);
+  // N(n) goes to the fourth slot, so no slot is read and written in one pass
+  const long m = it - 4;
   int rc = run_rpass(h, rbase(h, RP_NL | (check ? RP_W : 0)));
   if (rc) return rc;
   KPassArgs a = kbase(h);
   a.mode = KP_FWD | KP_INV | KP_ITER | conj_mode(h) | (check ? KP_STAT : 0);
   a.epi = EPI_AB4;
   a.dt = dt;
-  a.visc_kind = visc_kind_plain(h);  // solver.c:315-330
-  a.h_old = h->hist[m % 3];
-  a.h_mid = h->hist[(m + 1) % 3];
-  a.h_new = h->hist[(m + 2) % 3];
+  a.lam = lam_table(h, visc_kind_plain(h));  // solver.c:315-330
+  if (!a.lam) return fail(NS2D_ERR_HIP, "coefficient table allocation failed");
+  a.h_old = h->hist[m % 4];
+  a.h_mid = h->hist[(m + 1) % 4];
+  a.h_new = h->hist[(m + 2) % 4];
+  a.h_store = h->hist[(m + 3) % 4];
   a.conj_src = h->conj[h->cur];
   a.conj_dst = h->conj[h->cur ^ 1];
+  a.w_out = h->w[h->wcur ^ 1];
   rc = run_kpass(h, a);
   if (rc) return rc;
   h->cur ^= 1;
+  h->wcur ^= 1;
   h->iters = it;
   return NS2D_OK;
 }
@@ -270,6 +342,38 @@ extern "C" {
 
 const char* ns2d_last_error(void) { return g_err.c_str(); }
 
+int ns2d_variant_count(int pass, int n) {
+  if (pass == NS2D_PASS_COLUMN) return kpass_variant_count(n);
+  if (pass == NS2D_PASS_ROW) return rpass_variant_count(n);
+  return 0;
+}
+
+int ns2d_variant_info(int pass, int n, int idx, int* shape4) {
+  if (!shape4) return fail(NS2D_ERR_INVALID, "null argument");
+  int rc = -1;
+  if (pass == NS2D_PASS_COLUMN) rc = kpass_variant_info(n, idx, &shape4[0], &shape4[1], &shape4[2], &shape4[3]);
+  if (pass == NS2D_PASS_ROW) rc = rpass_variant_info(n, idx, &shape4[0], &shape4[1], &shape4[2], &shape4[3]);
+  if (rc) return fail(NS2D_ERR_INVALID, "no variant %d of pass %d for n=%d", idx, pass, n);
+  return NS2D_OK;
+}
+
+int ns2d_set_variant(ns2d_t* h, int pass, int idx) {
+  if (!h) return fail(NS2D_ERR_INVALID, "null handle");
+  if (pass == NS2D_PASS_COLUMN) {
+    if (idx < 0 || idx >= kpass_variant_count(h->p.nx))
+      return fail(NS2D_ERR_INVALID, "no column-pass variant %d for Nx=%d", idx, h->p.nx);
+    h->kvar = idx;
+    return NS2D_OK;
+  }
+  if (pass == NS2D_PASS_ROW) {
+    if (idx < 0 || idx >= rpass_variant_count(h->p.ny))
+      return fail(NS2D_ERR_INVALID, "no row-pass variant %d for Ny=%d", idx, h->p.ny);
+    h->rvar = idx;
+    return NS2D_OK;
+  }
+  return fail(NS2D_ERR_INVALID, "unknown pass %d", pass);
+}
+
 int ns2d_params_default(ns2d_params* p, int nx, int ny) {
   if (!p) return fail(NS2D_ERR_INVALID, "null params");
   std::memset(p, 0, sizeof *p);
@@ -293,7 +397,11 @@ int ns2d_params_default(ns2d_params* p, int nx, int ny) {
   return NS2D_OK;
 }
 
-int ns2d_create(const ns2d_params* p, ns2d_t** out) {
+static int create_impl(const ns2d_params* p, hipStream_t ext_stream, ns2d_t** out);
+
+int ns2d_create(const ns2d_params* p, ns2d_t** out) { return create_impl(p, nullptr, out); }
+
+static int create_impl(const ns2d_params* p, hipStream_t ext_stream, ns2d_t** out) {
   if (!p || !out) return fail(NS2D_ERR_INVALID, "null argument");
   *out = nullptr;
   if (!pow2_in_range(p->nx) || !pow2_in_range(p->ny))
@@ -342,12 +450,17 @@ int ns2d_create(const ns2d_params* p, ns2d_t** out) {
     ns2d_destroy(h);
     return rc;
   };
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+  if (ext_stream) {
+    h->stream = ext_stream;
+    h->own_stream = false;
+  } else if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     return cleanup(fail(NS2D_ERR_HIP, "stream create failed"));
+  }
   const size_t bytes = h->M * sizeof(double2);
-  std::vector<double2**> bufs = {&h->w_hat, &h->acc, &h->g[0], &h->g[1], &h->g[2], &h->g[3], &h->nbuf, &h->kout, &h->sbuf};
+  std::vector<double2**> bufs = {&h->w[0], &h->w[1], &h->acc, &h->g[0], &h->g[1], &h->g[2], &h->g[3],
+                                 &h->nbuf, &h->kout, &h->sbuf};
   if (p->integrator == NS2D_INT_AB4)
-    for (int i = 0; i < 3; ++i) bufs.push_back(&h->hist[i]);
+    for (int i = 0; i < 4; ++i) bufs.push_back(&h->hist[i]);
   for (double2** b : bufs) {
     if (hipMalloc((void**)b, bytes) != hipSuccess)
       return cleanup(fail(NS2D_ERR_HIP, "hipMalloc of %zu bytes failed", bytes));
@@ -372,25 +485,63 @@ int ns2d_create(const ns2d_params* p, ns2d_t** out) {
     return cleanup(fail(NS2D_ERR_HIP, "hipMalloc failed"));
   if (hipMemcpy(h->tw, tw.data(), sizeof(double2) * NS2D_TW_LEN, hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(fail(NS2D_ERR_HIP, "twiddle upload failed"));
+  // Hou-Li filter in its real-division form: per-mode table (solver.c:1088)
+  ph.ld = h->ld;
+  if (p->filter == NS2D_FILTER_HOULI) {
+    std::vector<double> t((size_t)p->nx * h->ld, 0.0);
+    for (int i = 0; i < p->nx; ++i) {
+      const long kx = i <= p->nx / 2 ? i : i - p->nx;
+      for (int j = 0; j < h->nyf; ++j)
+        t[(size_t)i * h->ld + j] = std::exp(-36.0 * std::pow(std::sqrt(std::pow(kx / (p->nx / 2.0), 2.0) +
+                                                                        std::pow(j / (p->ny / 2.0), 2.0)), 36.0));
+    }
+    if (hipMalloc((void**)&h->filt_tab, sizeof(double) * t.size()) != hipSuccess ||
+        hipMemcpy(h->filt_tab, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(NS2D_ERR_HIP, "filter table upload failed"));
+    ph.filt_tab = h->filt_tab;
+  }
+  if (const char* g = std::getenv("NS2D_GRAPH")) h->use_graph = std::atoi(g) != 0;
+  // pass shapes: built-in defaults, or NS2D_KPASS_VARIANT_<Nx> / NS2D_RPASS_VARIANT_<Ny>
+  {
+    char name[64];
+    std::snprintf(name, sizeof name, "NS2D_KPASS_VARIANT_%d", p->nx);
+    const char* env = std::getenv(name);
+    if (env) {
+      const int v = std::atoi(env);
+      if (v >= 0 && v < kpass_variant_count(p->nx)) h->kvar = v;
+    }
+    std::snprintf(name, sizeof name, "NS2D_RPASS_VARIANT_%d", p->ny);
+    env = std::getenv(name);
+    if (env) {
+      const int v = std::atoi(env);
+      if (v >= 0 && v < rpass_variant_count(p->ny)) h->rvar = v;
+    }
+  }
   int rc = reset_stat(h);
   if (rc) return cleanup(rc);
   *out = h;
   return NS2D_OK;
 }
 
+static void drop_graph(ns2d_handle* h);
+
 int ns2d_destroy(ns2d_t* h) {
   if (!h) return NS2D_OK;
   (void)hipSetDevice(h->p.device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  double2* bufs[] = {h->w_hat, h->acc, h->g[0], h->g[1], h->g[2], h->g[3], h->nbuf, h->kout, h->sbuf,
-                     h->hist[0], h->hist[1], h->hist[2], h->conj[0], h->conj[1], h->tw};
+  double2* bufs[] = {h->w[0], h->w[1], h->acc, h->g[0], h->g[1], h->g[2], h->g[3], h->nbuf, h->kout, h->sbuf,
+                     h->hist[0], h->hist[1], h->hist[2], h->hist[3], h->conj[0], h->conj[1], h->tw};
   for (double2* b : bufs)
     if (b) (void)hipFree(b);
+  for (double* t : h->lam_tab)
+    if (t) (void)hipFree(t);
+  if (h->filt_tab) (void)hipFree(h->filt_tab);
   if (h->stat) (void)hipFree(h->stat);
   if (h->meas) (void)hipFree(h->meas);
   if (h->wreal) (void)hipFree(h->wreal);
   if (h->rbuf) (void)hipFree(h->rbuf);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
+  drop_graph(h);
+  if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return NS2D_OK;
 }
@@ -398,8 +549,8 @@ int ns2d_destroy(ns2d_t* h) {
 static int set_state_impl(ns2d_t* h, const void* src, hipMemcpyKind kind) {
   if (!h || !src) return fail(NS2D_ERR_INVALID, "null argument");
   HIPCHK(hipSetDevice(h->p.device));
-  HIPCHK(hipMemcpy2DAsync(h->w_hat, h->ld * sizeof(double2), src, h->nyf * sizeof(double2), h->nyf * sizeof(double2),
-                          h->p.nx, kind, h->stream));
+  HIPCHK(hipMemcpy2DAsync(h->w[h->wcur], h->ld * sizeof(double2), src, h->nyf * sizeof(double2),
+                          h->nyf * sizeof(double2), h->p.nx, kind, h->stream));
   h->inv_valid = false;
   h->iters = 0;
   h->hyper_eff = h->p.hyper_flag;
@@ -409,8 +560,8 @@ static int set_state_impl(ns2d_t* h, const void* src, hipMemcpyKind kind) {
 static int get_state_impl(ns2d_t* h, void* dst, hipMemcpyKind kind) {
   if (!h || !dst) return fail(NS2D_ERR_INVALID, "null argument");
   HIPCHK(hipSetDevice(h->p.device));
-  HIPCHK(hipMemcpy2DAsync(dst, h->nyf * sizeof(double2), h->w_hat, h->ld * sizeof(double2), h->nyf * sizeof(double2),
-                          h->p.nx, kind, h->stream));
+  HIPCHK(hipMemcpy2DAsync(dst, h->nyf * sizeof(double2), h->w[h->wcur], h->ld * sizeof(double2),
+                          h->nyf * sizeof(double2), h->p.nx, kind, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   return NS2D_OK;
 }
@@ -419,6 +570,64 @@ int ns2d_set_state(ns2d_t* h, const double* w_hat) { return set_state_impl(h, w_
 int ns2d_get_state(ns2d_t* h, double* w_hat) { return get_state_impl(h, w_hat, hipMemcpyDeviceToHost); }
 int ns2d_set_state_device(ns2d_t* h, const void* d) { return set_state_impl(h, d, hipMemcpyDeviceToDevice); }
 int ns2d_get_state_device(ns2d_t* h, void* d) { return get_state_impl(h, d, hipMemcpyDeviceToDevice); }
+
+static void drop_graph(ns2d_handle* h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+}
+
+// One direct step of the configured integrator.
+static int one_step(ns2d_handle* h, double dt, bool check) {
+  return (h->p.integrator == NS2D_INT_AB4) ? ab4_step(h, dt, check) : rk4cn_step(h, dt, check);
+}
+
+// Steps repeat with period 2 (RK4CN: the state buffers alternate) or 4 (AB4:
+// the four history slots rotate), after which every kernel argument is the
+// same again, so one period is captured once as a hipGraph and replayed.  The
+// capture holds kernel launches only.
+static int graph_steps(ns2d_handle* h, double dt, bool check, int period) {
+  const double* lam = lam_table(h, h->p.integrator == NS2D_INT_AB4 ? visc_kind_plain(h) : visc_kind_rk(h));
+  if (!lam) return fail(NS2D_ERR_HIP, "coefficient table allocation failed");
+  const int ring = h->p.integrator == NS2D_INT_AB4 ? (int)((h->iters + 1 - 4) % 4) : 0;
+  const bool same = h->gexec && h->gkey.dt == dt && h->gkey.wcur == h->wcur && h->gkey.cur == h->cur &&
+                    h->gkey.ring == ring && h->gkey.kvar == h->kvar && h->gkey.rvar == h->rvar &&
+                    h->gkey.check == (int)check && h->gkey.period == period && h->gkey.lam == lam;
+  if (!same) {
+    drop_graph(h);
+    const int cur0 = h->cur, wcur0 = h->wcur, hyp0 = h->hyper_eff;
+    const long it0 = h->iters;
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = NS2D_OK;
+    for (int k = 0; k < period && rc == NS2D_OK; ++k) rc = one_step(h, dt, check);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(h->stream, &g);
+    h->cur = cur0;
+    h->wcur = wcur0;
+    h->hyper_eff = hyp0;
+    h->iters = it0;
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (ec != hipSuccess) return fail(NS2D_ERR_HIP, "graph capture: %s", hipGetErrorString(ec));
+    h->graph = g;
+    HIPCHK(hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0));
+    h->gkey.dt = dt;
+    h->gkey.wcur = h->wcur;
+    h->gkey.cur = h->cur;
+    h->gkey.ring = ring;
+    h->gkey.kvar = h->kvar;
+    h->gkey.rvar = h->rvar;
+    h->gkey.check = check;
+    h->gkey.period = period;
+    h->gkey.lam = lam;
+  }
+  HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+  h->iters += period;  // cur and wcur are back to their values after a period
+  return NS2D_OK;
+}
 
 static int step_impl(ns2d_t* h, double dt, long nsteps) {
   if (!h) return fail(NS2D_ERR_INVALID, "null handle");
@@ -429,8 +638,21 @@ static int step_impl(ns2d_t* h, double dt, long nsteps) {
   const bool check = h->p.check_every != 0;
   int rc = prepare_inv(h);
   if (rc) return rc;
-  for (long s = 0; s < nsteps; ++s) {
-    rc = (h->p.integrator == NS2D_INT_AB4) ? ab4_step(h, dt, check) : rk4cn_step(h, dt, check);
+  const bool ab4 = h->p.integrator == NS2D_INT_AB4;
+  const int period = ab4 ? 4 : 2;
+  long s = 0;
+  while (s < nsteps) {
+    // graphs only once the integrator's arguments are periodic: after the AB4
+    // start-up and after the viscous-flag update of the first RK4CN step
+    const bool settled = !(ab4 && h->iters < 3) &&
+                         !((h->p.quirks & NS2D_QUIRK_VISC_ASSIGN) && h->p.drag_flag != 1 && h->hyper_eff != 1);
+    if (h->use_graph && !h->profiling && settled && nsteps - s >= period) {
+      rc = graph_steps(h, dt, check, period);
+      s += period;
+    } else {
+      rc = one_step(h, dt, check);
+      s += 1;
+    }
     if (rc) return rc;
   }
   // the state's transforms are prepared by the last k-space pass
@@ -439,7 +661,7 @@ static int step_impl(ns2d_t* h, double dt, long nsteps) {
     // SystemCheck on the final state (solver.c:2309)
     ev_begin(h, KIND_OTHER);
     RPassArgs a = rbase(h, RP_W);
-    hipError_t e = launch_rpass(h->p.nx, h->p.ny, a, h->stream);
+    hipError_t e = launch_rpass(h->p.nx, h->p.ny, h->rvar, a, h->stream);
     ev_end(h);
     if (e != hipSuccess) return fail(NS2D_ERR_HIP, "rpass launch: %s", hipGetErrorString(e));
     unsigned long long st[STAT_N];
@@ -497,13 +719,14 @@ int ns2d_measures(ns2d_t* h, int with_flux, ns2d_measures_t* m, double* enrg_spe
     if (h->p.quirks & NS2D_QUIRK_CONJ_GLOBAL) h->inv_valid = false;
   }
   MeasArgs a{};
-  a.w_hat = h->w_hat;
+  a.w_hat = h->w[h->wcur];
   a.rhs = with_flux ? h->kout : nullptr;
   a.out = h->meas;
   a.nx = h->p.nx;
   a.ny = h->p.ny;
   a.ld = h->ld;
-  a.visc_kind = visc_kind_plain(h);  // sys_msr.c:160-175
+  a.lam = lam_table(h, visc_kind_plain(h));  // sys_msr.c:160-175
+  if (!a.lam) return fail(NS2D_ERR_HIP, "coefficient table allocation failed");
   a.quirk_enst_diss = (h->p.quirks & NS2D_QUIRK_ENST_DISS) ? 1 : 0;
   a.sbst_lo = 0.0;                   // LWR_SBST_LIM^2, data_types.h:175
   a.sbst_hi = 64.0 * 64.0;           // UPR_SBST_LIM^2, data_types.h:174
@@ -514,7 +737,7 @@ int ns2d_measures(ns2d_t* h, int with_flux, ns2d_measures_t* m, double* enrg_spe
   HIPCHK(hipMemcpyAsync(part.data(), h->meas, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));
   double2 wf = make_double2(0.0, 0.0);
   if (h->ph.force_on)
-    HIPCHK(hipMemcpyAsync(&wf, h->w_hat + h->p.force_k, sizeof wf, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(&wf, h->w[h->wcur] + h->p.force_k, sizeof wf, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   const double norm_fac = 0.5 / std::pow((double)((long)h->p.nx * h->p.ny), 2.0);  // sys_msr.c:43
   const double const_fac = 4.0 * std::pow(M_PI, 2.0);
@@ -658,6 +881,268 @@ int ns2d_profile_steps(ns2d_t* h, double dt, long nsteps, double* kpass_ms, doub
   if (rpass_ms) *rpass_ms = rn ? rs / rn : 0.0;
   if (total_ms) *total_ms = tot;
   return rc;
+}
+
+
+// ---------------------------------------------------------------------------
+// Ensemble: many independent solvers of possibly different sizes stepped
+// together.  Members of one grid size form a class; every pass of a stage is
+// one batched launch per class reading the members' arguments from a device
+// table, and one period of two steps (all classes) is replayed as a hipGraph.
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct ns2d_ensemble {
+  hipStream_t stream = nullptr;
+  std::vector<ns2d_handle*> m;
+  std::vector<double> dt;
+  struct Class {
+    int nx = 0, ny = 0, nyf = 0, kvar = 0, rvar = 0;
+    std::vector<int> idx;
+    KPassArgs* dk = nullptr;   // [parity 2][stage 4][members]
+    RPassArgs* dr = nullptr;   // [stage 4][members]
+  };
+  std::vector<Class> cls;
+  bool tables_ok = false;
+  int tables_parity = -1;
+  bool tables_check = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_parity = -1;
+};
+
+namespace {
+
+void ens_drop(ns2d_ensemble* e) {
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->gexec = nullptr;
+  e->graph = nullptr;
+  e->graph_parity = -1;
+}
+
+// (Re)build the per-class argument tables for the members' current buffers.
+int ens_tables(ns2d_ensemble* e, bool check) {
+  const int par = e->m[0]->wcur;
+  if (e->tables_ok && e->tables_parity == par && e->tables_check == check) return NS2D_OK;
+  ens_drop(e);
+  for (auto& c : e->cls) {
+    const size_t nm = c.idx.size();
+    std::vector<KPassArgs> hk(2 * 4 * nm);
+    std::vector<RPassArgs> hr(4 * nm);
+    for (size_t j = 0; j < nm; ++j) {
+      ns2d_handle* h = e->m[c.idx[j]];
+      for (int p2 = 0; p2 < 2; ++p2) {
+        const int w0 = h->wcur;
+        h->wcur = par ^ p2;
+        for (int st = 0; st < 4; ++st) {
+          RPassArgs ra;
+          KPassArgs ka;
+          int rc = rk4cn_stage_args(h, e->dt[c.idx[j]], check, st, &ra, &ka);
+          h->wcur = w0;
+          if (rc) return rc;
+          h->wcur = par ^ p2;
+          hk[((size_t)p2 * 4 + st) * nm + j] = ka;
+          if (p2 == 0) hr[(size_t)st * nm + j] = ra;
+        }
+        h->wcur = w0;
+      }
+    }
+    if (!c.dk) HIPCHK(hipMalloc((void**)&c.dk, sizeof(KPassArgs) * hk.size()));
+    if (!c.dr) HIPCHK(hipMalloc((void**)&c.dr, sizeof(RPassArgs) * hr.size()));
+    HIPCHK(hipMemcpy(c.dk, hk.data(), sizeof(KPassArgs) * hk.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.dr, hr.data(), sizeof(RPassArgs) * hr.size(), hipMemcpyHostToDevice));
+  }
+  e->tables_ok = true;
+  e->tables_parity = par;
+  e->tables_check = check;
+  return NS2D_OK;
+}
+
+// One batched RK4CN step of every member; p2 = parity relative to the tables.
+int ens_launch_step(ns2d_ensemble* e, int p2) {
+  for (int st = 0; st < 4; ++st) {
+    for (auto& c : e->cls) {
+      hipError_t r = launch_rpass_batched(c.nx, c.ny, c.rvar, c.dr + (size_t)st * c.idx.size(), (int)c.idx.size(),
+                                          e->stream);
+      if (r != hipSuccess) return fail(NS2D_ERR_HIP, "batched rpass: %s", hipGetErrorString(r));
+    }
+    for (auto& c : e->cls) {
+      hipError_t r = launch_kpass_batched(c.nx, c.nyf, c.kvar, c.dk + ((size_t)p2 * 4 + st) * c.idx.size(),
+                                          (int)c.idx.size(), e->stream);
+      if (r != hipSuccess) return fail(NS2D_ERR_HIP, "batched kpass: %s", hipGetErrorString(r));
+    }
+  }
+  return NS2D_OK;
+}
+
+void ens_advance(ns2d_ensemble* e) {
+  for (ns2d_handle* h : e->m) {
+    after_rk_update(h);
+    h->iters += 1;
+    h->wcur ^= 1;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ns2d_ens_create(const ns2d_params* members, int n, ns2d_ens_t** out) {
+  if (!members || !out || n <= 0) return fail(NS2D_ERR_INVALID, "bad ensemble arguments");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i)
+    if (members[i].integrator != NS2D_INT_RK4CN)
+      return fail(NS2D_ERR_INVALID, "ensemble member %d: only RK4CN members are batched", i);
+  HIPCHK(hipSetDevice(members[0].device));
+  ns2d_ensemble* e = new ns2d_ensemble();
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return fail(NS2D_ERR_HIP, "stream create failed");
+  }
+  for (int i = 0; i < n; ++i) {
+    ns2d_t* h = nullptr;
+    int rc = create_impl(&members[i], e->stream, &h);
+    if (rc) {
+      ns2d_ens_destroy(e);
+      return rc;
+    }
+    e->m.push_back(h);
+    e->dt.push_back(0.0);
+    int ci = -1;
+    for (size_t c = 0; c < e->cls.size(); ++c)
+      if (e->cls[c].nx == members[i].nx && e->cls[c].ny == members[i].ny) ci = (int)c;
+    if (ci < 0) {
+      ns2d_ensemble::Class c;
+      c.nx = members[i].nx;
+      c.ny = members[i].ny;
+      c.nyf = h->nyf;
+      c.kvar = h->kvar;
+      c.rvar = h->rvar;
+      e->cls.push_back(c);
+      ci = (int)e->cls.size() - 1;
+    }
+    e->cls[ci].idx.push_back(i);
+  }
+  *out = e;
+  return NS2D_OK;
+}
+
+int ns2d_ens_member(ns2d_ens_t* e, int i, ns2d_t** h) {
+  if (!e || !h || i < 0 || i >= (int)e->m.size()) return fail(NS2D_ERR_INVALID, "no ensemble member %d", i);
+  *h = e->m[i];
+  return NS2D_OK;
+}
+
+int ns2d_ens_step(ns2d_ens_t* e, const double* dt, long nsteps) {
+  if (!e || !dt || nsteps < 0) return fail(NS2D_ERR_INVALID, "bad ensemble step arguments");
+  HIPCHK(hipSetDevice(e->m[0]->p.device));
+  bool dt_changed = false;
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    if (!(dt[i] > 0.0)) return fail(NS2D_ERR_INVALID, "member %zu: dt must be positive", i);
+    if (dt[i] != e->dt[i]) dt_changed = true;
+    e->dt[i] = dt[i];
+  }
+  if (dt_changed) e->tables_ok = false;
+  if (nsteps == 0) return NS2D_OK;
+  const bool check = e->m[0]->p.check_every != 0;
+  for (ns2d_handle* h : e->m) {
+    int rc = prepare_inv(h);
+    if (rc) return rc;
+    if (h->wcur != e->m[0]->wcur) return fail(NS2D_ERR_STATE, "ensemble members out of step");
+  }
+  long s = 0;
+  // first steps one member at a time until the viscous flags have settled (Q3)
+  while (s < nsteps) {
+    bool settled = true;
+    for (ns2d_handle* h : e->m)
+      settled = settled && !((h->p.quirks & NS2D_QUIRK_VISC_ASSIGN) && h->p.drag_flag != 1 && h->hyper_eff != 1);
+    if (settled) break;
+    for (size_t i = 0; i < e->m.size(); ++i) {
+      int rc = rk4cn_step(e->m[i], e->dt[i], check);
+      if (rc) return rc;
+    }
+    e->tables_ok = false;
+    ++s;
+  }
+  if (s < nsteps) {
+    int rc = ens_tables(e, check);
+    if (rc) return rc;
+  }
+  while (s < nsteps) {
+    const int p2 = e->m[0]->wcur ^ e->tables_parity;
+    if (nsteps - s >= 2 && p2 == 0) {
+      if (!e->gexec) {
+        HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        int rc = ens_launch_step(e, 0);
+        if (rc == NS2D_OK) rc = ens_launch_step(e, 1);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(e->stream, &g);
+        if (rc) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc;
+        }
+        if (ec != hipSuccess) return fail(NS2D_ERR_HIP, "ensemble graph capture: %s", hipGetErrorString(ec));
+        e->graph = g;
+        HIPCHK(hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0));
+      }
+      HIPCHK(hipGraphLaunch(e->gexec, e->stream));
+      ens_advance(e);
+      ens_advance(e);
+      s += 2;
+    } else {
+      int rc = ens_launch_step(e, p2);
+      if (rc) return rc;
+      ens_advance(e);
+      s += 1;
+    }
+  }
+  for (ns2d_handle* h : e->m) h->inv_valid = true;
+  if (check) {
+    // SystemCheck of every member's final state (solver.c:2309)
+    for (ns2d_handle* h : e->m) {
+      RPassArgs a = rbase(h, RP_W);
+      hipError_t r = launch_rpass(h->p.nx, h->p.ny, h->rvar, a, e->stream);
+      if (r != hipSuccess) return fail(NS2D_ERR_HIP, "rpass launch: %s", hipGetErrorString(r));
+    }
+    int bad = -1;
+    for (size_t i = 0; i < e->m.size(); ++i) {
+      ns2d_handle* h = e->m[i];
+      unsigned long long st[STAT_N];
+      int rc = read_stat(h, st);
+      if (rc) return rc;
+      double mx;
+      std::memcpy(&mx, &st[STAT_WMAX_CUR], sizeof mx);
+      if (!(mx < MAX_VORT_LIM) && st[STAT_FIRST_BAD] == ~0ull) st[STAT_FIRST_BAD] = st[STAT_ITER];
+      st[STAT_WMAX_LAST] = st[STAT_WMAX_CUR];
+      st[STAT_WMAX_CUR] = 0ull;
+      HIPCHK(hipMemcpyAsync(h->stat, st, sizeof st, hipMemcpyHostToDevice, e->stream));
+      if (st[STAT_FIRST_BAD] != ~0ull && bad < 0) bad = (int)i;
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (bad >= 0) return fail(NS2D_ERR_BLOWUP, "ensemble member %d reached the maximum vorticity limit", bad);
+  }
+  return NS2D_OK;
+}
+
+int ns2d_ens_sync(ns2d_ens_t* e) {
+  if (!e) return fail(NS2D_ERR_INVALID, "null ensemble");
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return NS2D_OK;
+}
+
+int ns2d_ens_destroy(ns2d_ens_t* e) {
+  if (!e) return NS2D_OK;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  ens_drop(e);
+  for (auto& c : e->cls) {
+    if (c.dk) (void)hipFree(c.dk);
+    if (c.dr) (void)hipFree(c.dr);
+  }
+  for (ns2d_handle* h : e->m) ns2d_destroy(h);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return NS2D_OK;
 }
 
 }  // extern "C"

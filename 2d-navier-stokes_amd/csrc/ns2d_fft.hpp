@@ -1,13 +1,17 @@
 // Stockham auto-sort FFT for FP64 complex lines resident in LDS (gfx950).
 //
 // A workgroup of T threads transforms NL lines of length L (power of two) that
-// sit in LDS at line stride LS (in double2 elements).  Each pass of radix R
-// (8 while at least 8 points remain, then 4 or 2) reads R points per
-// butterfly from LDS, applies the twiddles, does the R-point DFT in registers
-// and writes the outputs back to their auto-sorted positions, so the result
-// lands in natural order with no bit-reversal pass.  Twiddles come from one
-// table tw[m] = exp(-2*pi*i*m/NS2D_TW_LEN) built on the host in extended
-// precision; a pass with span P and radix R reads entry r*k*(NS2D_TW_LEN/(P*R)).
+// sit in LDS at line stride LS (double2 elements).  Element j of a line is
+// stored at padx(j) = j + j/8: one pad slot every 8 elements keeps the strided
+// writes of the early passes (stride R elements) on distinct LDS banks.
+//
+// Each pass of radix R (8 while at least 8 points remain, then 4 or 2) reads R
+// points per butterfly from LDS, applies the twiddles, does the R-point DFT in
+// registers and writes the outputs to their auto-sorted positions, so the
+// result lands in natural order with no bit-reversal pass.  Twiddles come from
+// one table tw[m] = exp(-2*pi*i*m/NS2D_TW_LEN) built on the host in extended
+// precision: a butterfly with span P and offset k loads w = tw[k*NS2D_TW_LEN/(P*R)]
+// once and forms w^r by repeated multiplication.
 //
 // SIGN = -1 is the forward transform (e^{-i}), +1 the inverse (e^{+i});
 // neither is normalised, matching FFTW's conventions.
@@ -32,6 +36,12 @@ template <int SIGN>
 __device__ __forceinline__ double2 cmul_si(double2 a) {
   return SIGN < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
 }
+
+// padded LDS position of element j of a line
+__host__ __device__ constexpr int padx(int j) { return j + (j >> 3); }
+// LDS line stride for lines of length L: padded length, odd so that
+// neighbouring lines start on different banks
+__host__ __device__ constexpr int line_stride(int L) { return padx(L) + 1; }
 
 template <int SIGN>
 __device__ __forceinline__ void dft2(double2* v) {
@@ -102,9 +112,9 @@ __device__ __forceinline__ void stockham_pass(double2* lds, const double2* __res
     const int b = tid + it * T;
     if ((TOT % T == 0) || b < TOT) {
       const int line = b / NBF, i = b % NBF;
-      const double2* src = lds + line * LS + i;
+      const double2* src = lds + line * LS;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[it][r] = src[r * NBF];
+      for (int r = 0; r < R; ++r) v[it][r] = src[padx(i + r * NBF)];
     }
   }
   __syncthreads();
@@ -115,13 +125,19 @@ __device__ __forceinline__ void stockham_pass(double2* lds, const double2* __res
       const int line = b / NBF, i = b % NBF;
       const int k = i & (P - 1);
       if constexpr (P > 1) {
+        const double2 w = twiddle<SIGN>(tw, k * TWS);
+        double2 wr = w;
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[it][r] = cmul(v[it][r], twiddle<SIGN>(tw, r * k * TWS));
+        for (int r = 1; r < R; ++r) {
+          v[it][r] = cmul(v[it][r], wr);
+          if (r + 1 < R) wr = cmul(wr, w);
+        }
       }
       dft<R, SIGN>(v[it]);
-      double2* dst = lds + line * LS + (i - k) * R + k;
+      double2* dst = lds + line * LS;
+      const int j0 = (i - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[r * P] = v[it][r];
+      for (int r = 0; r < R; ++r) dst[padx(j0 + r * P)] = v[it][r];
     }
   }
   __syncthreads();
@@ -137,9 +153,9 @@ __device__ __forceinline__ void fft_passes(double2* lds, const double2* __restri
   }
 }
 
-// Transform NL lines of length L in LDS in place (natural order in and out).
-// Requires a barrier before the call if the data was just written by other
-// threads; ends with a barrier.
+// Transform NL lines of length L in LDS in place (natural order in and out,
+// padded positions).  Requires a barrier before the call if the data was just
+// written by other threads; ends with a barrier.
 template <int L, int NL, int LS, int T, int SIGN>
 __device__ __forceinline__ void fft_lds(double2* lds, const double2* __restrict__ tw, int tid) {
   static_assert((L & (L - 1)) == 0 && L >= 2, "power-of-two length");

@@ -1,0 +1,158 @@
+// Register-resident Stockham FFT for FP64 complex lines (gfx950).
+//
+// A line of N points (power of two) is owned by TL threads; thread l holds the
+// E = N/TL points x[l + TL*e], e = 0..E-1 (the "stride-TL distribution").
+// Each pass of radix R = min(16, E, remaining) runs E/R butterflies per thread
+// entirely in registers: butterfly i = l + TL*m reads x[i + r*N/R], which is
+// register m + r*E/R of the same thread.  Between passes the outputs are
+// written to their Stockham positions in an LDS line buffer and read back in
+// the stride-TL distribution (one exchange per pass boundary).  The outputs of
+// the last pass land in natural order in the same stride-TL distribution, so a
+// transform starts and ends in registers: an N = 4096 line with E = 16 costs
+// three radix-16 passes and two LDS exchanges.
+//
+// LDS positions are padded (padx: one slot every 8 elements) so that the
+// stride-R writes of the first pass hit distinct banks.  Twiddles: one load
+// w = tw[k*NS2D_TW_LEN/(P*R)] per butterfly from the host-built table
+// exp(-2*pi*i*m/NS2D_TW_LEN); the powers w^r are formed by multiplication
+// (apply_twiddles below).
+//
+// SIGN = -1: forward (e^{-i}); +1: inverse (e^{+i}); unnormalised, as FFTW.
+// BLOCK selects the exchange barrier: __syncthreads() when a line's threads
+// span several waves, a wavefront-scope fence otherwise.
+#pragma once
+#include "ns2d_fft.hpp"
+
+namespace ns2d {
+
+template <bool BLOCK>
+__device__ __forceinline__ void xsync() {
+  if constexpr (BLOCK) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// 16-point DFT as 4 x 4: n = n1 + 4 n2, k = k1 + 4 k2
+template <int SIGN>
+__device__ __forceinline__ void dft16(double2* v) {
+  constexpr double C1 = 0.92387953251128675612818318939679;  // cos(pi/8)
+  constexpr double S1 = 0.38268343236508977172845998403040;  // sin(pi/8)
+  constexpr double H = 0.70710678118654752440084436210485;   // cos(pi/4)
+  double2 y[4][4];
+#pragma unroll
+  for (int n1 = 0; n1 < 4; ++n1) {
+    double2 t[4] = {v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]};
+    dft4<SIGN>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) y[n1][k1] = t[k1];
+  }
+  // y[n1][k1] *= W16^(n1*k1), W16 = exp(SIGN*2*pi*i/16)
+  auto rot = [](double2 a, double c, double s) {  // a * (c + SIGN*i*s)
+    return SIGN < 0 ? make_double2(a.x * c + a.y * s, a.y * c - a.x * s)
+                    : make_double2(a.x * c - a.y * s, a.y * c + a.x * s);
+  };
+  y[1][1] = rot(y[1][1], C1, S1);    // W^1
+  y[1][2] = rot(y[1][2], H, H);      // W^2
+  y[1][3] = rot(y[1][3], S1, C1);    // W^3
+  y[2][1] = rot(y[2][1], H, H);      // W^2
+  y[2][2] = cmul_si<SIGN>(y[2][2]);  // W^4
+  y[2][3] = rot(y[2][3], -H, H);     // W^6
+  y[3][1] = rot(y[3][1], S1, C1);    // W^3
+  y[3][2] = rot(y[3][2], -H, H);     // W^6
+  y[3][3] = rot(y[3][3], -C1, -S1);  // W^9
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    double2 t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+    dft4<SIGN>(t);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+  }
+}
+
+template <int R, int SIGN>
+__device__ __forceinline__ void dftr(double2* v) {
+  if constexpr (R == 16) dft16<SIGN>(v);
+  else if constexpr (R == 1) {}
+  else dft<R, SIGN>(v);
+}
+
+// v[r] *= w^r for r = 1..R-1, with w^(4a+b) = (w^4)^a w^b so that only
+// w, w^2, w^3 and the running (w^4)^a are live (at most five roundings deep)
+template <int R>
+__device__ __forceinline__ void apply_twiddles(double2* v, double2 w) {
+  if constexpr (R == 2) {
+    v[1] = cmul(v[1], w);
+  } else {
+    const double2 w2 = cmul(w, w);
+    const double2 w3 = cmul(w2, w);
+    v[1] = cmul(v[1], w);
+    v[2] = cmul(v[2], w2);
+    v[3] = cmul(v[3], w3);
+    if constexpr (R > 4) {
+      const double2 w4 = cmul(w2, w2);
+      double2 base = w4;
+#pragma unroll
+      for (int a = 1; a < R / 4; ++a) {
+        v[4 * a] = cmul(v[4 * a], base);
+        v[4 * a + 1] = cmul(v[4 * a + 1], cmul(base, w));
+        v[4 * a + 2] = cmul(v[4 * a + 2], cmul(base, w2));
+        v[4 * a + 3] = cmul(v[4 * a + 3], cmul(base, w3));
+        if (a + 1 < R / 4) base = cmul(base, w4);
+      }
+    }
+  }
+}
+
+template <int N, int TL, int SIGN, bool BLOCK, int P>
+__device__ __forceinline__ void fft_reg_pass(double2* v, double2* buf, const double2* __restrict__ tw, int l) {
+  constexpr int E = N / TL;
+  constexpr int REM = N / P;
+  if constexpr (REM > 1) {
+    constexpr int R0 = E < 16 ? E : 16;
+    constexpr int R = R0 < REM ? R0 : REM;
+    constexpr int NB = E / R;          // butterflies per thread
+    constexpr bool LAST = (P * R == N);
+    constexpr int TWS = NS2D_TW_LEN / (P * R);
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+      const int i = l + TL * m;
+      double2 x[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[r] = v[m + r * NB];
+      const int k = i & (P - 1);
+      if constexpr (P > 1) apply_twiddles<R>(x, twiddle<SIGN>(tw, k * TWS));
+      dftr<R, SIGN>(x);
+      if constexpr (LAST) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[m + r * NB] = x[r];
+      } else {
+        const int j0 = (i - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[padx(j0 + r * P)] = x[r];
+      }
+    }
+    if constexpr (!LAST) {
+      xsync<BLOCK>();
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = buf[padx(l + TL * e)];
+      xsync<BLOCK>();
+      fft_reg_pass<N, TL, SIGN, BLOCK, P * R>(v, buf, tw, l);
+    }
+  }
+}
+
+// In-register FFT of one line: v holds x[l + TL*e] on entry and X[l + TL*e]
+// on exit.  buf: this line's LDS exchange buffer (padx(N) elements).  All TL
+// threads of the line (and, with BLOCK, every thread of the block) must call.
+template <int N, int TL, int SIGN, bool BLOCK>
+__device__ __forceinline__ void fft_reg(double2* v, double2* buf, const double2* __restrict__ tw, int l) {
+  static_assert((N & (N - 1)) == 0 && N >= 2 && N % TL == 0, "bad FFT shape");
+  static_assert(N <= NS2D_TW_LEN, "twiddle table too short");
+  fft_reg_pass<N, TL, SIGN, BLOCK, 1>(v, buf, tw, l);
+}
+
+}  // namespace ns2d

@@ -45,15 +45,19 @@ struct PhysArgs {
   double norm2;         // (1/(Nx*Ny))^2, solver.c:1005
   int force_on, force_k;
   double force_val;
+  int ld;
+  const double* filt_tab;  // FILTER_HOULI: per-mode filter, [Nx][ld]
 };
 
 struct KPassArgs {
-  double2* w_hat;
+  const double2* w_in;   // state read by the epilogue
+  double2* w_out;        // new state (RK4CN stage 4, AB4) / ForceConjugacy target
   double2* acc;
-  double2* h_old;
-  double2* h_mid;
-  double2* h_new;
-  double2* hist_store;
+  const double2* h_old;
+  const double2* h_mid;
+  const double2* h_new;
+  double2* h_store;      // AB4: slot receiving this step's nonlinear term
+  double2* hist_store;   // AB4 start-up: K1 of an RK4CN step
   const double2* nin;
   const double2* sin;
   double2* g[4];
@@ -63,13 +67,15 @@ struct KPassArgs {
   unsigned long long* stat;
   const double2* tw;
   int nyf, ld;
-  int mode, epi, visc_kind;
+  const double* lam;     // per-mode D_fac/dt of the CN update, [Nx][ld]
+  int mode, epi;
   double dt, coef;
   PhysArgs ph;
 };
 
 struct RPassArgs {
   const double2* g[4];
+  int nx;
   double2* nout;
   double* wreal;
   unsigned long long* wmax;
@@ -82,7 +88,8 @@ struct MeasArgs {
   const double2* w_hat;
   const double2* rhs;
   double* out;
-  int nx, ny, ld, visc_kind, quirk_enst_diss;
+  const double* lam;     // per-mode pre_fac (sys_msr.c:160-175), [Nx][ld]
+  int nx, ny, ld, quirk_enst_diss;
   double sbst_lo, sbst_hi;
   PhysArgs ph;
 };
@@ -97,8 +104,14 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
   return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + j;
 }
 
-hipError_t launch_kpass(int nx, const KPassArgs& a, hipStream_t s);
-hipError_t launch_rpass(int nx, int ny, const RPassArgs& a, hipStream_t s);
+hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s);
+int kpass_variant_count(int nx);
+hipError_t launch_kpass_batched(int nx, int nyf, int variant, const KPassArgs* tab, int members, hipStream_t s);
+hipError_t launch_rpass_batched(int nx, int ny, int variant, const RPassArgs* tab, int members, hipStream_t s);
+int kpass_variant_info(int nx, int idx, int* C, int* E, int* T, int* KS);
+hipError_t launch_rpass(int nx, int ny, int variant, const RPassArgs& a, hipStream_t s);
+int rpass_variant_count(int ny);
+int rpass_variant_info(int ny, int idx, int* TLR, int* E, int* T, int* PAIRS);
 hipError_t launch_r2c_rows(int nx, int ny, const double* in, double2* out, const double2* tw, int ld,
                            hipStream_t s);
 hipError_t launch_measures(int nbins, const MeasArgs& a, hipStream_t s);

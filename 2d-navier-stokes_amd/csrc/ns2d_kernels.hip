@@ -27,30 +27,23 @@
 #include <stdint.h>
 
 #include "ns2d_fft.hpp"
+#include "ns2d_fft_reg.hpp"
 #include "ns2d_internal.h"
 
 namespace ns2d {
 
 __device__ __forceinline__ int kx_of_row(int i, int nx) { return i <= nx / 2 ? i : i - nx; }
 
-// pow(x, e) with the exponents the solver uses evaluated exactly
-__device__ __forceinline__ double pow_k2(double x, double e) {
-  if (e == 1.0) return x;
-  if (e == 2.0) return x * x;
-  if (e == 0.0) return 1.0;
-  return pow(x, e);
+// Returns v unchanged but opaque to the optimiser.  Used at the top of the
+// per-field / per-batch loops so that index and twiddle arithmetic is redone in
+// each iteration instead of being hoisted and kept live across the loop (which
+// otherwise exhausts the register file and spills to scratch).
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
 }
 
-// D_fac / dt of the CN update for |k|^2 = k2 (k2 > 0), kind chosen on the host
-// from the reference's flag logic (solver.c:835-850 / :315-330).
-__device__ __forceinline__ double visc_coeff(const PhysArgs& ph, int kind, double k2) {
-  switch (kind) {
-    case VISC_HYPER_DRAG: return ph.nu * pow_k2(k2, ph.hyper_pow) + ph.alpha * pow_k2(k2, ph.drag_pow);
-    case VISC_K2_DRAG: return ph.nu * k2 + ph.alpha * pow_k2(k2, ph.drag_pow);
-    case VISC_HYPER: return ph.nu * pow_k2(k2, ph.hyper_pow);
-    default: return ph.nu * k2;
-  }
-}
+
 
 // ApplyDealiasing multiplier, solver.c:1039-1099
 __device__ __forceinline__ double dealias_factor(const PhysArgs& ph, int kx, int ky) {
@@ -67,145 +60,191 @@ __device__ __forceinline__ double dealias_factor(const PhysArgs& ph, int kx, int
       const int q = qx * qx + qy * qy;
       return q == 0 ? 1.0 : (q == 1 ? ph.houli_edge : ph.houli_corner);
     }
-    case FILTER_HOULI: {
-      const double a = (double)kx / (ph.nx / 2.0), b = (double)ky / (ph.ny / 2.0);
-      const double r2 = a * a + b * b;
-      const double r4 = r2 * r2, r8 = r4 * r4, r16 = r8 * r8, r32 = r16 * r16;
-      return exp(-36.0 * (r32 * r4));  // (sqrt(r2))^36
-    }
+    case FILTER_HOULI:   // exp(-36 (|k|/k_N)^36), tabulated on the host
+      return ph.filt_tab[(kx < 0 ? kx + ph.nx : kx) * ph.ld + ky];
     default: return 1.0;
   }
 }
 
 // ---------------------------------------------------------------------------
 // kpass: column pass in k-space
+//
+// A work item is a group of C adjacent ky-columns.  Thread tid owns column
+// c = tid % C and the rows x = l + TL*e (l = tid / C, TL = T/C, e < E = L/TL),
+// so each global access of a wave covers C contiguous complex values of a row,
+// and the column's x-transform runs in registers (fft_reg) with the column's
+// own LDS exchange buffer.  KS workgroups share a column group: each redoes the
+// forward transform and takes 4/KS of the inverse fields; only split 0 writes
+// the integrator state (the new state goes to a separate buffer w_out, so no
+// workgroup reads what another writes).
 // ---------------------------------------------------------------------------
-template <int L, int C, int T>
-__global__ __launch_bounds__(T) void kpass_kernel(KPassArgs a) {
-  constexpr int LS = (C > 1) ? L + 1 : L;   // padded line stride in LDS
-  constexpr int NE = C * L;                 // elements handled by the workgroup
-  constexpr int IT = (NE + T - 1) / T;
+template <int L, int C, int T, int KS>
+__device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
+  constexpr int TL = T / C;        // threads per column
+  constexpr int E = L / TL;        // rows per thread
+  constexpr int LS = line_stride(L);
+  constexpr int FPS = 4 / KS;      // inverse fields per split
+  constexpr bool BLOCK = T > 64;
+  static_assert(T % C == 0 && L % TL == 0 && E >= 2 && 4 % KS == 0, "bad kpass shape");
   __shared__ double2 lds[C * LS];
 
   const int tid = threadIdx.x;
-  const int g = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int g = wi / KS, split = wi % KS;
   const int ky0 = g * C;
+  const int c = tid % C, l = tid / C;
+  const int ky = ky0 + c;
   const int nyf = a.nyf, ld = a.ld, nx = L;
+  const bool colv = ky < nyf;
+  const bool writer = split == 0;
   const double2* __restrict__ tw = a.tw;
   const PhysArgs& ph = a.ph;
+  double2* buf = lds + c * LS;
 
-  double2 S[IT];     // next stage input (registers), element e = tid + it*T
-  double2 K[IT];
+  double2 R[E];  // forward result K, then the next stage input S
 
   // ---------------- FWD: x-forward FFT of the row-pass output -------------
   if (a.mode & KP_FWD) {
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int e = tid + it * T;
-      if ((NE % T == 0) || e < NE) {
-        const int c = e % C, x = e / C, ky = ky0 + c;
-        lds[c * LS + x] = (ky < nyf) ? a.nin[(size_t)x * ld + ky] : make_double2(0.0, 0.0);
-      }
+    for (int e = 0; e < E; ++e) {
+      const int x = l + TL * e;
+      R[e] = colv ? a.nin[(size_t)x * ld + ky] : make_double2(0.0, 0.0);
+      if ((e & 3) == 3) asm volatile("" ::: "memory");
     }
-    __syncthreads();
-    fft_lds<L, C, LS, T, -1>(lds, tw, tid);
+    fft_reg<L, TL, -1, BLOCK>(R, buf, tw, opaque(l));
     const double nrm = ph.norm2;
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int e = tid + it * T;
-      if ((NE % T == 0) || e < NE) {
-        const int c = e % C, i = e / C, ky = ky0 + c;
-        const int kx = kx_of_row(i, nx);
-        double2 v = lds[c * LS + i];
-        if (a.mode & KP_RAW) {
-          // plain unnormalised forward transform (initial conditions)
-        } else if (kx == 0 && ky == 0) {
-          v = make_double2(0.0, 0.0);
-        } else {
-          v = cscale(nrm, v);                                  // solver.c:1005
-          v = cscale(dealias_factor(ph, kx, ky), v);           // solver.c:1019
-          if (ph.force_on && kx == 0 && ky == ph.force_k)      // solver.c:1022-1027
-            v.x += ph.force_val;
-        }
-        K[it] = v;
+    for (int e = 0; e < E; ++e) {
+      const int kx = kx_of_row(l + TL * e, nx);
+      double2 v = R[e];
+      if (a.mode & KP_RAW) {
+        // plain unnormalised forward transform (initial conditions)
+      } else if (kx == 0 && ky == 0) {
+        v = make_double2(0.0, 0.0);
+      } else {
+        v = cscale(nrm, v);                                  // solver.c:1005
+        v = cscale(dealias_factor(ph, kx, ky), v);           // solver.c:1019
+        if (ph.force_on && kx == 0 && ky == ph.force_k)      // solver.c:1022-1027
+          v.x += ph.force_val;
       }
+      R[e] = v;
     }
   }
 
   // ---------------- EPI: integrator stage ----------------------------------
-  const int epi = a.epi;
-  const bool conj_fix = (a.mode & KP_CONJFIX) && ky0 == 0;
-#pragma unroll
-  for (int it = 0; it < IT; ++it) {
-    const int e = tid + it * T;
-    if (!((NE % T == 0) || e < NE)) continue;
-    const int c = e % C, i = e / C, ky = ky0 + c;
-    if (ky >= nyf) { S[it] = make_double2(0.0, 0.0); continue; }
-    const size_t idx = (size_t)i * ld + ky;
+  // The state w as the epilogue sees it, with the ForceConjugacy write of the
+  // RHS just finished: w_hat[kx<0, 0] = conj(stage_input[|kx|, 0])
+  // (solver.c:2096-2102).  Padding columns (ky >= Ny/2+1) carry zeros.
+  const bool conj_fix = (a.mode & KP_CONJFIX) && ky0 == 0 && c == 0;
+  auto state_at = [&](int it) -> double2 {
+    const int i = l + TL * it;
     const int kx = kx_of_row(i, nx);
-    if (epi == EPI_FROM_STATE) { S[it] = a.w_hat[idx]; continue; }
-    if (epi == EPI_FROM_INPUT) { S[it] = a.sin[idx]; continue; }
-    // the state, with the ForceConjugacy write of the RHS just finished:
-    // w_hat[kx<0, 0] = conj(stage_input[|kx|, 0])  (solver.c:2096-2102)
-    double2 w;
-    const bool fixed = conj_fix && c == 0 && kx < 0;
-    if (fixed) w = cconj(a.conj_src[-kx]);
-    else if (epi != EPI_RHS) w = a.w_hat[idx];
-    const double2 k = K[it];
-    switch (epi) {
-      case EPI_RHS:
-        a.kout[idx] = k;
-        if (fixed) a.w_hat[idx] = w;
-        break;
-      case EPI_RK1:
-        a.acc[idx] = cscale(RK4_B1, k);
-        if (a.hist_store) a.hist_store[idx] = k;
-        S[it] = cadd(w, cscale(a.coef, k));                  // solver.c:690
-        break;
-      case EPI_RK2:
-      case EPI_RK3: {
-        const double b = (epi == EPI_RK2) ? RK4_B2 : RK4_B3;
-        a.acc[idx] = cadd(a.acc[idx], cscale(b, k));
-        S[it] = cadd(w, cscale(a.coef, k));                  // solver.c:729,768
-        break;
+    if (conj_fix && kx < 0) return cconj(a.conj_src[-kx]);
+    return a.w_in[(size_t)i * ld + ky];
+  };
+  auto index_of = [&](int it) -> size_t { return (size_t)(l + TL * it) * ld + ky; };
+  switch (a.epi) {
+    case EPI_FROM_STATE:
+#pragma unroll
+      for (int it = 0; it < E; ++it) {
+        R[it] = colv ? a.w_in[index_of(it)] : make_double2(0.0, 0.0);
+        if ((it & 3) == 3) asm volatile("" ::: "memory");
       }
-      case EPI_RK4CN: {
-        double2 nw;
-        if (kx == 0 && ky == 0) {
-          nw = make_double2(0.0, 0.0);                         // solver.c:860
-        } else {
-          const double2 sum = cadd(a.acc[idx], cscale(RK4_B4, k));
-          const double D = a.dt * visc_coeff(ph, a.visc_kind, (double)(kx * kx + ky * ky));
-          nw = cadd(cscale((2.0 - D) / (2.0 + D), w), cscale(2.0 * a.dt / (2.0 + D), sum));  // :853
+      break;
+    case EPI_FROM_INPUT:
+#pragma unroll
+      for (int it = 0; it < E; ++it) {
+        R[it] = colv ? a.sin[index_of(it)] : make_double2(0.0, 0.0);
+        if ((it & 3) == 3) asm volatile("" ::: "memory");
+      }
+      break;
+    case EPI_RHS:
+      if (writer && colv) {
+#pragma unroll
+        for (int it = 0; it < E; ++it) {
+          a.kout[index_of(it)] = R[it];
+          const int kx = kx_of_row(l + TL * it, nx);
+          if (conj_fix && kx < 0) a.w_out[index_of(it)] = cconj(a.conj_src[-kx]);
         }
-        a.w_hat[idx] = nw;
-        S[it] = nw;
-        break;
       }
-      case EPI_AB4: {
-        double2 nw;
-        if (kx == 0 && ky == 0) {
-          nw = make_double2(0.0, 0.0);                         // solver.c:337
-        } else {
-          const double2 h2 = a.h_new[idx], h1 = a.h_mid[idx], h0 = a.h_old[idx];
-          const double2 sum = cadd(cadd(cadd(cscale(AB4_1, k), cscale(AB4_2, h2)), cscale(AB4_3, h1)),
-                                   cscale(AB4_4, h0));
-          const double D = a.dt * visc_coeff(ph, a.visc_kind, (double)(kx * kx + ky * ky));
-          nw = cadd(cscale((2.0 - D) / (2.0 + D), w), cscale(2.0 * a.dt / (2.0 + D), sum));  // :333
+      break;
+    case EPI_RK1:
+      if (colv) {
+#pragma unroll
+        for (int it = 0; it < E; ++it) {
+          const double2 k = R[it];
+          if (writer) {
+            a.acc[index_of(it)] = cscale(RK4_B1, k);
+            if (a.hist_store) a.hist_store[index_of(it)] = k;
+          }
+          R[it] = cadd(state_at(it), cscale(a.coef, k));     // solver.c:690
+          if ((it & 3) == 3) asm volatile("" ::: "memory");
         }
-        a.h_old[idx] = k;   // the oldest history slot becomes the newest (:346-348)
-        a.w_hat[idx] = nw;
-        S[it] = nw;
-        break;
       }
-      default: break;
-    }
+      break;
+    case EPI_RK2:
+    case EPI_RK3:
+      if (colv) {
+        const double b = (a.epi == EPI_RK2) ? RK4_B2 : RK4_B3;
+#pragma unroll
+        for (int it = 0; it < E; ++it) {
+          const double2 k = R[it];
+          if (writer) a.acc[index_of(it)] = cadd(a.acc[index_of(it)], cscale(b, k));
+          R[it] = cadd(state_at(it), cscale(a.coef, k));     // solver.c:729,768
+          if ((it & 3) == 3) asm volatile("" ::: "memory");
+        }
+      }
+      break;
+    case EPI_RK4CN:
+      if (colv) {
+#pragma unroll
+        for (int it = 0; it < E; ++it) {
+          const int kx = kx_of_row(l + TL * it, nx);
+          double2 nw = make_double2(0.0, 0.0);               // (0,0) mode: solver.c:860
+          if (kx != 0 || ky != 0) {
+            const double2 sum = cadd(a.acc[index_of(it)], cscale(RK4_B4, R[it]));
+            const double D = a.dt * a.lam[index_of(it)];
+            nw = cadd(cscale((2.0 - D) / (2.0 + D), state_at(it)), cscale(2.0 * a.dt / (2.0 + D), sum));  // :853
+          }
+          if (writer) a.w_out[index_of(it)] = nw;
+          R[it] = nw;
+          if ((it & 3) == 3) asm volatile("" ::: "memory");
+        }
+      }
+      break;
+    case EPI_AB4:
+      if (colv) {
+#pragma unroll
+        for (int it = 0; it < E; ++it) {
+          const int kx = kx_of_row(l + TL * it, nx);
+          const size_t idx = index_of(it);
+          const double2 k = R[it];
+          double2 nw = make_double2(0.0, 0.0);               // (0,0) mode: solver.c:337
+          if (kx != 0 || ky != 0) {
+            const double2 sum = cadd(cadd(cadd(cscale(AB4_1, k), cscale(AB4_2, a.h_new[idx])),
+                                          cscale(AB4_3, a.h_mid[idx])), cscale(AB4_4, a.h_old[idx]));
+            const double D = a.dt * a.lam[idx];
+            nw = cadd(cscale((2.0 - D) / (2.0 + D), state_at(it)), cscale(2.0 * a.dt / (2.0 + D), sum));  // :333
+          }
+          if (writer) {
+            a.h_store[idx] = k;   // newest history entry (solver.c:346-348)
+            a.w_out[idx] = nw;
+          }
+          R[it] = nw;
+          if ((it & 3) == 3) asm volatile("" ::: "memory");
+        }
+      }
+      break;
+    default: break;
+  }
+  if (!colv) {
+#pragma unroll
+    for (int it = 0; it < E; ++it) R[it] = make_double2(0.0, 0.0);
   }
 
   // blow-up bookkeeping for the reference's per-step SystemCheck
   // (solver.c:2301-2326): the stage-1 row pass left max|w| of the state that
   // this step started from in stat[0]; record it and reset the accumulator.
-  if ((a.mode & KP_STAT) && blockIdx.x == 0 && tid == 0) {
+  if ((a.mode & KP_STAT) && wi == 0 && tid == 0) {
     unsigned long long* st = a.stat;
     const unsigned long long m = st[STAT_WMAX_CUR];
     st[STAT_WMAX_LAST] = m;
@@ -213,194 +252,203 @@ __global__ __launch_bounds__(T) void kpass_kernel(KPassArgs a) {
       st[STAT_FIRST_BAD] = st[STAT_ITER];
     st[STAT_WMAX_CUR] = 0ull;
   }
-  if ((a.mode & KP_ITER) && blockIdx.x == 0 && tid == 0) a.stat[STAT_ITER] += 1ull;
+  if ((a.mode & KP_ITER) && wi == 0 && tid == 0) a.stat[STAT_ITER] += 1ull;
 
   if (!(a.mode & KP_INV)) return;
 
   // ky=0 column of the next stage input, for the next ForceConjugacy
-  if (ky0 == 0 && a.conj_dst) {
+  if (writer && ky0 == 0 && c == 0 && a.conj_dst) {
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int e = tid + it * T;
-      if ((NE % T == 0) || e < NE) {
-        if (e % C == 0) a.conj_dst[e / C] = S[it];
-      }
-    }
+    for (int it = 0; it < E; ++it) a.conj_dst[l + TL * it] = R[it];
   }
 
-  // ---------------- INV: four x-inverse FFTs --------------------------------
+  // ---------------- INV: x-inverse FFTs of the derived fields ---------------
 #pragma unroll 1
-  for (int f = 0; f < 4; ++f) {
-    __syncthreads();
+  for (int fi = 0; fi < FPS; ++fi) {
+    const int f = split * FPS + fi;
+    const int l_o = opaque(l), ky_o = opaque(ky);
+    double2 F[E];
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int e = tid + it * T;
-      if ((NE % T == 0) || e < NE) {
-        const int c = e % C, i = e / C, ky = ky0 + c;
-        const int kx = kx_of_row(i, nx);
-        const int k2i = kx * kx + ky * ky;
-        const double inv = k2i ? 1.0 / (double)k2i : 0.0;
-        const double2 s = S[it];
-        double2 v;
-        if (f == 0) v = cscale(inv, s);                                      // psi_hat
-        else if (f == 1) v = make_double2(kx * inv * s.y, -(kx * inv) * s.x);  // -i kx psi_hat
-        else if (f == 2) v = make_double2(-kx * s.y, kx * s.x);              // i kx w_hat
-        else v = s;                                                          // w_hat
-        lds[c * LS + i] = v;
-      }
+    for (int e = 0; e < E; ++e) {
+      const int kx = kx_of_row(l_o + TL * e, nx);
+      const int k2i = kx * kx + ky_o * ky_o;
+      const double inv = k2i ? 1.0 / (double)k2i : 0.0;
+      const double2 s = R[e];
+      if (f == 0) F[e] = cscale(inv, s);                                      // psi_hat
+      else if (f == 1) F[e] = make_double2(kx * inv * s.y, -(kx * inv) * s.x);  // -i kx psi_hat
+      else if (f == 2) F[e] = make_double2(-kx * s.y, kx * s.x);              // i kx w_hat
+      else F[e] = s;                                                          // w_hat
     }
-    __syncthreads();
-    fft_lds<L, C, LS, T, +1>(lds, tw, tid);
-    double2* __restrict__ gout = a.g[f];
+    fft_reg<L, TL, +1, BLOCK>(F, buf, tw, l_o);
+    if (colv) {
+      double2* __restrict__ gout = a.g[f];
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int e = tid + it * T;
-      if ((NE % T == 0) || e < NE) {
-        const int c = e % C, x = e / C, ky = ky0 + c;
-        if (ky < nyf) gout[(size_t)x * ld + ky] = lds[c * LS + x];
-      }
+      for (int e = 0; e < E; ++e) gout[(size_t)(l_o + TL * e) * ld + ky_o] = F[e];
     }
   }
 }
 
+template <int L, int C, int T, int KS>
+__global__ __launch_bounds__(T) void kpass_kernel(KPassArgs a) {
+  kpass_body<L, C, T, KS>(a, xcd_remap((int)blockIdx.x, (int)gridDim.x));
+}
+
+// All members of an ensemble size class in one launch: work items
+// [m * per_member, (m+1) * per_member) belong to member m, whose arguments
+// are tab[m] (uniform per workgroup, read with scalar loads).
+template <int L, int C, int T, int KS>
+__global__ __launch_bounds__(T) void kpass_batched(const KPassArgs* __restrict__ tab, int per_member) {
+  const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int m = b / per_member;
+  kpass_body<L, C, T, KS>(tab[m], b - m * per_member);
+}
+
 // ---------------------------------------------------------------------------
-// rpass: row pass in (x, ky) space
+// rpass: row pass in (x, ky) space, one row pair (x0, x0+1) per TLR threads
+//
+// Thread l of a pair owns y (or ky) = l + TLR*e, e < E = NY/TLR.  For each row
+// the full-length spectrum of (A + iB) is assembled from the stored half
+// spectra of two real fields (ky >= 0 read directly, ky < 0 as conjugates;
+// ky = 0 and NY/2 keep only their real parts), transformed in registers and
+// multiplied pointwise.  The two rows' products are packed as n(x0) + i n(x0+1)
+// and transformed forward once; N(x0,k) = (W[k] + conj W[NY-k])/2 and
+// N(x0+1,k) = (W[k] - conj W[NY-k])/(2i).
 // ---------------------------------------------------------------------------
-template <int NY, int T>
-__global__ __launch_bounds__(T) void rpass_kernel(RPassArgs a) {
+template <int NY, int TLR, int PAIRS>
+__device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
+  constexpr int T = TLR * PAIRS;
   constexpr int H = NY / 2;
-  constexpr int ITY = (NY + T - 1) / T;     // y points per thread
-  constexpr int ITK = (H + 1 + T - 1) / T;  // ky points per thread
-  __shared__ double2 lds[NY];
-  __shared__ double red[T / 64];
+  constexpr int E = NY / TLR;
+  constexpr int LS = line_stride(NY);
+  constexpr bool BLOCK = TLR > 64;
+  static_assert(E >= 2 && (BLOCK ? PAIRS == 1 : 64 % TLR == 0), "bad rpass shape");
+  __shared__ double2 lds[PAIRS * LS];
+  __shared__ double red[(T + 63) / 64];
 
   const int tid = threadIdx.x;
-  const int x = blockIdx.x;
+  const int q = tid / TLR, l = tid % TLR;
+  const int x0 = 2 * (blk * PAIRS + q);
+  const bool valid = x0 < a.nx;   // uniform per block when BLOCK
   const int ld = a.ld;
   const double2* __restrict__ tw = a.tw;
-  const size_t row = (size_t)x * ld;
+  double2* buf = lds + q * LS;
 
-  double n[ITY];
-  double2 z1[ITY];
-
-  // builds the full-length spectrum of (A + i B) from the half spectra of two
-  // real fields; the ky=0 and ky=NY/2 entries keep only their real parts
-  auto build_pair = [&](int which) {
+  // full-length spectrum of A + iB for row x (which: 0 -> u + iv, 1 -> w_x + i w_y,
+  // 2 -> w(x) + i w(x+1))
+  auto build = [&](double2* z, int which, int x, int lo) {
 #pragma unroll
-    for (int it = 0; it < ITK; ++it) {
-      const int ky = tid + it * T;
-      if (ky <= H) {
-        double2 A, B;
-        const double fky = (double)ky;
-        if (which == 0) {         // u = i ky G1, v = G2
-          const double2 g1 = a.g[0][row + ky];
-          A = make_double2(-fky * g1.y, fky * g1.x);
-          B = a.g[1][row + ky];
-        } else if (which == 1) {  // w_x = G3, w_y = i ky G4
-          A = a.g[2][row + ky];
-          const double2 g4 = a.g[3][row + ky];
-          B = make_double2(-fky * g4.y, fky * g4.x);
-        } else {                  // w = G4, paired with zero
-          A = a.g[3][row + ky];
-          B = make_double2(0.0, 0.0);
-        }
-        if (ky == 0 || ky == H) { A.y = 0.0; B.y = 0.0; }
-        lds[ky] = make_double2(A.x - B.y, A.y + B.x);
-        if (ky > 0 && ky < H) lds[NY - ky] = make_double2(A.x + B.y, B.x - A.y);
+    for (int e = 0; e < E; ++e) {
+      const int ky = lo + TLR * e;
+      const bool mirror = ky > H;
+      const int kk = mirror ? NY - ky : ky;
+      double2 A, B;
+      const double fk = (double)kk;
+      if (which == 0) {         // u = i ky G1, v = G2
+        const double2 g1 = a.g[0][(size_t)x * ld + kk];
+        A = make_double2(-fk * g1.y, fk * g1.x);
+        B = a.g[1][(size_t)x * ld + kk];
+      } else if (which == 1) {  // w_x = G3, w_y = i ky G4
+        A = a.g[2][(size_t)x * ld + kk];
+        const double2 g4 = a.g[3][(size_t)x * ld + kk];
+        B = make_double2(-fk * g4.y, fk * g4.x);
+      } else {                  // w of rows x and x+1 = G4
+        A = a.g[3][(size_t)x * ld + kk];
+        B = a.g[3][(size_t)(x + 1) * ld + kk];
       }
+      if (kk == 0 || kk == H) { A.y = 0.0; B.y = 0.0; }
+      z[e] = mirror ? make_double2(A.x + B.y, B.x - A.y) : make_double2(A.x - B.y, A.y + B.x);
+      if ((e & 3) == 3) asm volatile("" ::: "memory");
     }
   };
 
-  if (a.mode & RP_NL) {
-    build_pair(0);
-    __syncthreads();
-    fft_lds<NY, 1, NY, T, +1>(lds, tw, tid);
+  double n0[E];
+  double2 w[E];
+  if ((a.mode & RP_NL) && (BLOCK || valid)) {
+#pragma unroll 1
+    for (int r = 0; r < 2; ++r) {
+      const int x = opaque(x0 + r);
+      double2 z1[E], z2[E];
+      build(z1, 0, x, opaque(l));
+      fft_reg<NY, TLR, +1, BLOCK>(z1, buf, tw, opaque(l));
+      build(z2, 1, x, opaque(l));
+      fft_reg<NY, TLR, +1, BLOCK>(z2, buf, tw, opaque(l));
 #pragma unroll
-    for (int it = 0; it < ITY; ++it) {
-      const int y = tid + it * T;
-      if (y < NY) z1[it] = lds[y];
-    }
-    __syncthreads();
-    build_pair(1);
-    __syncthreads();
-    fft_lds<NY, 1, NY, T, +1>(lds, tw, tid);
-#pragma unroll
-    for (int it = 0; it < ITY; ++it) {
-      const int y = tid + it * T;
-      if (y < NY) {
-        const double2 z2 = lds[y];
-        n[it] = z1[it].x * z2.x + z1[it].y * z2.y;   // u*w_x + v*w_y
+      for (int e = 0; e < E; ++e) {
+        const double nl = z1[e].x * z2[e].x + z1[e].y * z2[e].y;   // u*w_x + v*w_y
+        if (r == 0) n0[e] = nl;
+        else w[e] = make_double2(n0[e], nl);
       }
     }
   }
 
   if (a.mode & RP_W) {
-    __syncthreads();
-    build_pair(2);
-    __syncthreads();
-    fft_lds<NY, 1, NY, T, +1>(lds, tw, tid);
     double m = 0.0;
+    if (BLOCK || valid) {
+      double2 z[E];
+      build(z, 2, x0, opaque(l));
+      fft_reg<NY, TLR, +1, BLOCK>(z, buf, tw, opaque(l));
 #pragma unroll
-    for (int it = 0; it < ITY; ++it) {
-      const int y = tid + it * T;
-      if (y < NY) {
-        const double w = lds[y].x * a.wscale;
-        if (a.wreal) a.wreal[(size_t)x * NY + y] = w;
-        const double aw = fabs(w);
+      for (int e = 0; e < E; ++e) {
+        const int y = l + TLR * e;
+        const double w0 = z[e].x * a.wscale, w1 = z[e].y * a.wscale;
+        if (a.wreal) {
+          a.wreal[(size_t)x0 * NY + y] = w0;
+          a.wreal[(size_t)(x0 + 1) * NY + y] = w1;
+        }
+        const double aw = fmax(fabs(w0), fabs(w1));
         m = (aw > m || aw != aw) ? aw : m;
+        if (w0 != w0 || w1 != w1) m = __longlong_as_double(0x7ff8000000000000ll);
       }
     }
     // wave then block max; a NaN propagates (its bit pattern compares largest)
     unsigned long long mb = (unsigned long long)__double_as_longlong(m);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-      unsigned long long o = __shfl_xor(mb, off, 64);
+      const unsigned long long o = __shfl_xor(mb, off, 64);
       mb = o > mb ? o : mb;
     }
     if ((tid & 63) == 0) red[tid >> 6] = __longlong_as_double((long long)mb);
     __syncthreads();
     if (tid == 0) {
       unsigned long long bm = 0;
-      for (int w = 0; w < T / 64; ++w) {
-        unsigned long long v = (unsigned long long)__double_as_longlong(red[w]);
+      for (int i = 0; i < (T + 63) / 64; ++i) {
+        const unsigned long long v = (unsigned long long)__double_as_longlong(red[i]);
         bm = v > bm ? v : bm;
       }
       atomicMax(a.wmax, bm);
     }
   }
 
-  if (a.mode & RP_NL) {
-    __syncthreads();
-    double* ldsr = reinterpret_cast<double*>(lds);
+  if ((a.mode & RP_NL) && (BLOCK || valid)) {
+    fft_reg<NY, TLR, -1, BLOCK>(w, buf, tw, opaque(l));
+    // W[NY-k] for the thread's own k: exchange through the line buffer
 #pragma unroll
-    for (int it = 0; it < ITY; ++it) {
-      const int y = tid + it * T;
-      if (y < NY) ldsr[y] = n[it];
-    }
-    __syncthreads();
-    // packed c[m] = n[2m] + i n[2m+1]; C = FFT_H(c)
-    fft_lds<H, 1, H, T, -1>(lds, tw, tid);
-    constexpr int TWS = NS2D_TW_LEN / NY;
-    double2* __restrict__ out = a.nout + row;
+    for (int e = 0; e < E; ++e) buf[padx(l + TLR * e)] = w[e];
+    xsync<BLOCK>();
+    double2* __restrict__ out0 = a.nout + (size_t)x0 * ld;
+    double2* __restrict__ out1 = out0 + ld;
 #pragma unroll
-    for (int it = 0; it < ITK; ++it) {
-      const int k = tid + it * T;
+    for (int e = 0; e < E; ++e) {
+      const int k = l + TLR * e;
       if (k <= H) {
-        double2 X;
-        if (k == 0 || k == H) {
-          const double2 c0 = lds[0];
-          X = make_double2(k == 0 ? c0.x + c0.y : c0.x - c0.y, 0.0);
-        } else {
-          const double2 p = lds[k];
-          const double2 q = cconj(lds[H - k]);
-          const double2 E = make_double2(0.5 * (p.x + q.x), 0.5 * (p.y + q.y));
-          const double2 O = make_double2(0.5 * (p.y - q.y), -0.5 * (p.x - q.x));  // (p-q)/(2i)
-          X = cadd(E, cmul(tw[k * TWS], O));
-        }
-        out[k] = X;
+        const double2 p = w[e];
+        const double2 qm = cconj(buf[padx((NY - k) & (NY - 1))]);
+        out0[k] = make_double2(0.5 * (p.x + qm.x), 0.5 * (p.y + qm.y));
+        out1[k] = make_double2(0.5 * (p.y - qm.y), -0.5 * (p.x - qm.x));
       }
     }
   }
+}
+
+template <int NY, int TLR, int PAIRS>
+__global__ __launch_bounds__(TLR * PAIRS) void rpass_kernel(RPassArgs a) {
+  rpass_body<NY, TLR, PAIRS>(a, (int)blockIdx.x);
+}
+
+template <int NY, int TLR, int PAIRS>
+__global__ __launch_bounds__(TLR * PAIRS) void rpass_batched(const RPassArgs* __restrict__ tab, int per_member) {
+  const int m = (int)blockIdx.x / per_member;
+  rpass_body<NY, TLR, PAIRS>(tab[m], (int)blockIdx.x - m * per_member);
 }
 
 // ---------------------------------------------------------------------------
@@ -412,16 +460,16 @@ __global__ __launch_bounds__(T) void r2c_rows_kernel(const double* __restrict__ 
   constexpr int H = NY / 2;
   constexpr int ITY = (NY + T - 1) / T;
   constexpr int ITK = (H + 1 + T - 1) / T;
-  __shared__ double2 lds[H];
+  __shared__ double2 lds[line_stride(H)];
   const int tid = threadIdx.x, x = blockIdx.x;
   double* ldsr = reinterpret_cast<double*>(lds);
 #pragma unroll
   for (int it = 0; it < ITY; ++it) {
     const int y = tid + it * T;
-    if (y < NY) ldsr[y] = in[(size_t)x * NY + y];
+    if (y < NY) ldsr[2 * padx(y >> 1) + (y & 1)] = in[(size_t)x * NY + y];
   }
   __syncthreads();
-  fft_lds<H, 1, H, T, -1>(lds, tw, tid);
+  fft_lds<H, 1, line_stride(H), T, -1>(lds, tw, tid);
   constexpr int TWS = NS2D_TW_LEN / NY;
 #pragma unroll
   for (int it = 0; it < ITK; ++it) {
@@ -432,8 +480,8 @@ __global__ __launch_bounds__(T) void r2c_rows_kernel(const double* __restrict__ 
         const double2 c0 = lds[0];
         X = make_double2(k == 0 ? c0.x + c0.y : c0.x - c0.y, 0.0);
       } else {
-        const double2 p = lds[k];
-        const double2 q = cconj(lds[H - k]);
+        const double2 p = lds[padx(k)];
+        const double2 q = cconj(lds[padx(H - k)]);
         const double2 E = make_double2(0.5 * (p.x + q.x), 0.5 * (p.y + q.y));
         const double2 O = make_double2(0.5 * (p.y - q.y), -0.5 * (p.x - q.x));
         X = cadd(E, cmul(tw[k * TWS], O));
@@ -481,7 +529,7 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
         const double k2 = (double)(kx2 + ky * ky);
         const double c = (ky == 0 || ky == H) ? 1.0 : 2.0;
         const double a2 = w.x * w.x + w.y * w.y;
-        const double pre = visc_coeff(a.ph, a.visc_kind, k2);
+        const double pre = a.lam[idx];
         acc[Q_ENRG] += c * a2 / k2;
         acc[Q_ENST] += c * a2;
         acc[Q_PALIN] += c * k2 * a2;
@@ -535,68 +583,207 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
 // ---------------------------------------------------------------------------
 // host-side launchers (template dispatch on the transform lengths)
 // ---------------------------------------------------------------------------
-template <int L>
-struct KCfg {
-  static constexpr int C = L <= 256 ? 16 : (4096 / L);
-  static constexpr int T = (C * L / 4) < 256 ? ((C * L / 4) < 64 ? 64 : C * L / 4) : 256;
+// kpass variants {L, C, T, KS}: L-point columns, C columns per workgroup,
+// T threads, KS workgroups per column group; the first listed for each L is
+// the default.  Rows per thread E = L*C/T.
+#define NS2D_KPASS_VARIANTS(X) \
+  X(8, 16, 64, 1)              \
+  X(16, 16, 64, 1)             \
+  X(32, 16, 64, 1)             \
+  X(64, 16, 64, 1)             \
+  X(128, 16, 128, 1)           \
+  X(128, 8, 64, 2)             \
+  X(128, 8, 128, 2)            \
+  X(256, 16, 256, 1)           \
+  X(256, 8, 128, 2)            \
+  X(256, 8, 128, 4)            \
+  X(256, 8, 256, 2)            \
+  X(512, 8, 256, 2)            \
+  X(512, 8, 256, 1)            \
+  X(512, 4, 128, 4)            \
+  X(512, 8, 256, 4)            \
+  X(512, 8, 512, 2)            \
+  X(512, 4, 256, 2)            \
+  X(1024, 4, 256, 2)           \
+  X(1024, 4, 256, 1)           \
+  X(1024, 4, 256, 4)           \
+  X(1024, 2, 128, 2)           \
+  X(1024, 2, 128, 4)           \
+  X(1024, 1, 64, 4)            \
+  X(1024, 4, 512, 2)           \
+  X(1024, 2, 256, 2)           \
+  X(1024, 4, 512, 1)           \
+  X(2048, 2, 256, 2)           \
+  X(2048, 2, 256, 1)           \
+  X(2048, 4, 512, 1)           \
+  X(2048, 1, 128, 4)           \
+  X(2048, 2, 512, 1)           \
+  X(2048, 4, 1024, 1)          \
+  X(4096, 2, 512, 1)           \
+  X(4096, 1, 256, 1)           \
+  X(4096, 1, 256, 2)           \
+  X(4096, 2, 512, 2)           \
+  X(4096, 2, 1024, 1)          \
+  X(4096, 1, 512, 1)           \
+  X(4096, 1, 512, 2)
+
+struct KVariant {
+  int L, C, T, KS;
+  void (*kern)(KPassArgs);
+  void (*batched)(const KPassArgs*, int);
 };
 
-template <int L>
-static hipError_t launch_kpass_t(const KPassArgs& a, hipStream_t s) {
-  constexpr int C = KCfg<L>::C, T = KCfg<L>::T;
-  const int groups = (a.nyf + C - 1) / C;
-  hipLaunchKernelGGL((kpass_kernel<L, C, T>), dim3(groups), dim3(T), 0, s, a);
+#define NS2D_KV_ENTRY(L_, C_, T_, KS_) \
+  {L_, C_, T_, KS_, kpass_kernel<L_, C_, T_, KS_>, kpass_batched<L_, C_, T_, KS_>},
+static const KVariant kvariants[] = {NS2D_KPASS_VARIANTS(NS2D_KV_ENTRY)};
+#undef NS2D_KV_ENTRY
+
+static const KVariant* find_kvariant(int nx, int idx) {
+  int n = 0;
+  for (const KVariant& v : kvariants) {
+    if (v.L != nx) continue;
+    if (n++ == idx) return &v;
+  }
+  return nullptr;
+}
+
+// Every member of the batch must run a pass with inverse transforms (KS as
+// compiled) and share nyf.
+hipError_t launch_kpass_batched(int nx, int nyf, int variant, const KPassArgs* tab, int members, hipStream_t s) {
+  const KVariant* v = find_kvariant(nx, variant);
+  if (!v) return hipErrorInvalidValue;
+  const int per = (nyf + v->C - 1) / v->C * v->KS;
+  hipLaunchKernelGGL(v->batched, dim3(per * members), dim3(v->T), 0, s, tab, per);
   return hipGetLastError();
 }
 
-hipError_t launch_kpass(int nx, const KPassArgs& a, hipStream_t s) {
-  switch (nx) {
-    case 8: return launch_kpass_t<8>(a, s);
-    case 16: return launch_kpass_t<16>(a, s);
-    case 32: return launch_kpass_t<32>(a, s);
-    case 64: return launch_kpass_t<64>(a, s);
-    case 128: return launch_kpass_t<128>(a, s);
-    case 256: return launch_kpass_t<256>(a, s);
-    case 512: return launch_kpass_t<512>(a, s);
-    case 1024: return launch_kpass_t<1024>(a, s);
-    case 2048: return launch_kpass_t<2048>(a, s);
-    case 4096: return launch_kpass_t<4096>(a, s);
-    default: return hipErrorInvalidValue;
+int kpass_variant_count(int nx) {
+  int n = 0;
+  for (const KVariant& v : kvariants) n += v.L == nx;
+  return n;
+}
+
+int kpass_variant_info(int nx, int idx, int* C, int* E, int* T, int* KS) {
+  const KVariant* v = find_kvariant(nx, idx);
+  if (!v) return -1;
+  *C = v->C;
+  *E = v->L * v->C / v->T;
+  *T = v->T;
+  *KS = v->KS;
+  return 0;
+}
+
+// Launch the kpass for an nx-point column transform.  A pass without inverse
+// transforms (an RHS evaluation) uses a KS = 1 sibling of the same shape:
+// split workgroups exist only to share the inverse fields.
+hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s) {
+  const KVariant* v = find_kvariant(nx, variant);
+  if (!v) return hipErrorInvalidValue;
+  if (!(a.mode & KP_INV) && v->KS != 1) {
+    const KVariant* w = nullptr;
+    for (const KVariant& u : kvariants)
+      if (u.L == nx && u.KS == 1 && u.C == v->C && u.T == v->T) { w = &u; break; }
+    if (!w)
+      for (const KVariant& u : kvariants)
+        if (u.L == nx && u.KS == 1) { w = &u; break; }
+    if (!w) return hipErrorInvalidValue;
+    v = w;
   }
+  const int groups = (a.nyf + v->C - 1) / v->C;
+  hipLaunchKernelGGL(v->kern, dim3(groups * v->KS), dim3(v->T), 0, s, a);
+  return hipGetLastError();
+}
+
+// rpass variants {NY, TLR, PAIRS}: TLR threads per row pair (E = NY/TLR points
+// per thread), PAIRS row pairs per workgroup; the first listed for each NY is
+// the default.
+#define NS2D_RPASS_VARIANTS(X) \
+  X(8, 1, 64)                  \
+  X(16, 1, 64)                 \
+  X(32, 2, 32)                 \
+  X(32, 4, 16)                 \
+  X(64, 4, 16)                 \
+  X(64, 8, 8)                  \
+  X(128, 8, 8)                 \
+  X(128, 16, 4)                \
+  X(128, 32, 2)                \
+  X(256, 16, 4)                \
+  X(256, 32, 2)                \
+  X(256, 64, 1)                \
+  X(512, 32, 2)                \
+  X(512, 64, 1)                \
+  X(512, 128, 1)               \
+  X(1024, 64, 1)               \
+  X(1024, 128, 1)              \
+  X(1024, 256, 1)              \
+  X(2048, 128, 1)              \
+  X(2048, 256, 1)              \
+  X(2048, 512, 1)              \
+  X(4096, 256, 1)              \
+  X(4096, 512, 1)              \
+  X(4096, 1024, 1)
+
+struct RVariant {
+  int NY, TLR, PAIRS;
+  void (*kern)(RPassArgs);
+  void (*batched)(const RPassArgs*, int);
+};
+
+#define NS2D_RV_ENTRY(NY_, TLR_, PAIRS_) \
+  {NY_, TLR_, PAIRS_, rpass_kernel<NY_, TLR_, PAIRS_>, rpass_batched<NY_, TLR_, PAIRS_>},
+static const RVariant rvariants[] = {NS2D_RPASS_VARIANTS(NS2D_RV_ENTRY)};
+#undef NS2D_RV_ENTRY
+
+static const RVariant* find_rvariant(int ny, int idx) {
+  int n = 0;
+  for (const RVariant& v : rvariants) {
+    if (v.NY != ny) continue;
+    if (n++ == idx) return &v;
+  }
+  return nullptr;
+}
+
+hipError_t launch_rpass_batched(int nx, int ny, int variant, const RPassArgs* tab, int members, hipStream_t s) {
+  const RVariant* v = find_rvariant(ny, variant);
+  if (!v) return hipErrorInvalidValue;
+  const int per = (nx / 2 + v->PAIRS - 1) / v->PAIRS;
+  hipLaunchKernelGGL(v->batched, dim3(per * members), dim3(v->TLR * v->PAIRS), 0, s, tab, per);
+  return hipGetLastError();
+}
+
+int rpass_variant_count(int ny) {
+  int n = 0;
+  for (const RVariant& v : rvariants) n += v.NY == ny;
+  return n;
+}
+
+int rpass_variant_info(int ny, int idx, int* TLR, int* E, int* T, int* PAIRS) {
+  const RVariant* v = find_rvariant(ny, idx);
+  if (!v) return -1;
+  *TLR = v->TLR;
+  *E = v->NY / v->TLR;
+  *T = v->TLR * v->PAIRS;
+  *PAIRS = v->PAIRS;
+  return 0;
+}
+
+hipError_t launch_rpass(int nx, int ny, int variant, const RPassArgs& a, hipStream_t s) {
+  const RVariant* v = find_rvariant(ny, variant);
+  if (!v) return hipErrorInvalidValue;
+  const int pairs = nx / 2;
+  hipLaunchKernelGGL(v->kern, dim3((pairs + v->PAIRS - 1) / v->PAIRS), dim3(v->TLR * v->PAIRS), 0, s, a);
+  return hipGetLastError();
 }
 
 template <int NY>
-struct RCfg {
+struct R2CCfg {
   static constexpr int T = NY >= 1024 ? 256 : (NY / 4 < 64 ? 64 : NY / 4);
 };
 
 template <int NY>
-static hipError_t launch_rpass_t(int nx, const RPassArgs& a, hipStream_t s) {
-  constexpr int T = RCfg<NY>::T;
-  hipLaunchKernelGGL((rpass_kernel<NY, T>), dim3(nx), dim3(T), 0, s, a);
-  return hipGetLastError();
-}
-
-hipError_t launch_rpass(int nx, int ny, const RPassArgs& a, hipStream_t s) {
-  switch (ny) {
-    case 8: return launch_rpass_t<8>(nx, a, s);
-    case 16: return launch_rpass_t<16>(nx, a, s);
-    case 32: return launch_rpass_t<32>(nx, a, s);
-    case 64: return launch_rpass_t<64>(nx, a, s);
-    case 128: return launch_rpass_t<128>(nx, a, s);
-    case 256: return launch_rpass_t<256>(nx, a, s);
-    case 512: return launch_rpass_t<512>(nx, a, s);
-    case 1024: return launch_rpass_t<1024>(nx, a, s);
-    case 2048: return launch_rpass_t<2048>(nx, a, s);
-    case 4096: return launch_rpass_t<4096>(nx, a, s);
-    default: return hipErrorInvalidValue;
-  }
-}
-
-template <int NY>
 static hipError_t launch_r2c_rows_t(int nx, const double* in, double2* out, const double2* tw, int ld,
                                     hipStream_t s) {
-  constexpr int T = RCfg<NY>::T;
+  constexpr int T = R2CCfg<NY>::T;
   hipLaunchKernelGGL((r2c_rows_kernel<NY, T>), dim3(nx), dim3(T), 0, s, in, out, tw, ld);
   return hipGetLastError();
 }

@@ -63,7 +63,9 @@ class _Info(ctypes.Structure):
 EXPORTS = ["ns2d_params_default", "ns2d_create", "ns2d_destroy", "ns2d_set_state", "ns2d_get_state",
            "ns2d_set_state_device", "ns2d_get_state_device", "ns2d_step", "ns2d_rhs", "ns2d_measures",
            "ns2d_max_vort", "ns2d_real_vort", "ns2d_r2c", "ns2d_get_info", "ns2d_sync",
-           "ns2d_get_stream", "ns2d_profile_steps", "ns2d_last_error"]
+           "ns2d_get_stream", "ns2d_profile_steps", "ns2d_variant_count", "ns2d_variant_info",
+           "ns2d_set_variant", "ns2d_ens_create", "ns2d_ens_member", "ns2d_ens_step", "ns2d_ens_sync",
+           "ns2d_ens_destroy", "ns2d_last_error"]
 
 _lib = None
 
@@ -96,6 +98,14 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ns2d_sync": ([vp], i),
         "ns2d_get_stream": ([vp, P(vp)], i),
         "ns2d_profile_steps": ([vp, d, l, dp, dp, dp], i),
+        "ns2d_variant_count": ([i, i], i),
+        "ns2d_variant_info": ([i, i, i, P(i)], i),
+        "ns2d_set_variant": ([vp, i, i], i),
+        "ns2d_ens_create": ([P(_Params), i, P(vp)], i),
+        "ns2d_ens_member": ([vp, i, P(vp)], i),
+        "ns2d_ens_step": ([vp, dp, l], i),
+        "ns2d_ens_sync": ([vp], i),
+        "ns2d_ens_destroy": ([vp], i),
         "ns2d_last_error": ([], ctypes.c_char_p),
     }
     for name, (argt, rest) in sig.items():
@@ -104,6 +114,32 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         fn.restype = rest
     _lib = lib
     return lib
+
+
+PASS_COLUMN, PASS_ROW = 0, 1
+
+
+def variants(pass_kind: int, n: int) -> list:
+    """Compiled shapes of one pass for an n-point transform.  Column pass:
+    dicts with cols, per_thread, threads, splits; row pass: threads_per_pair,
+    per_thread, threads, pairs."""
+    lib = load_library()
+    keys = (("cols", "per_thread", "threads", "splits") if pass_kind == PASS_COLUMN
+            else ("threads_per_pair", "per_thread", "threads", "pairs"))
+    out = []
+    for i in range(lib.ns2d_variant_count(pass_kind, n)):
+        v = (ctypes.c_int * 4)()
+        _check(lib.ns2d_variant_info(pass_kind, n, i, v))
+        out.append(dict(zip(keys, list(v))))
+    return out
+
+
+def kpass_variants(nx: int) -> list:
+    return variants(PASS_COLUMN, nx)
+
+
+def rpass_variants(ny: int) -> list:
+    return variants(PASS_ROW, ny)
 
 
 class NS2DError(RuntimeError):
@@ -151,19 +187,23 @@ class Config:
 class Solver:
     """One solver instance on one GPU (owns device memory and a HIP stream)."""
 
-    def __init__(self, cfg: Config):
+    def __init__(self, cfg: Config, _handle=None):
         lib = load_library()
         self.cfg = cfg
         self.Nx, self.Ny, self.Nyf = cfg.Nx, cfg.Ny, cfg.Ny // 2 + 1
-        p = cfg._c()
-        h = ctypes.c_void_p()
-        _check(lib.ns2d_create(ctypes.byref(p), ctypes.byref(h)))
-        self._h = h
+        self._owned = _handle is None
+        if _handle is None:
+            p = cfg._c()
+            h = ctypes.c_void_p()
+            _check(lib.ns2d_create(ctypes.byref(p), ctypes.byref(h)))
+            _handle = h
+        self._h = _handle
         self.n_spect = self.info()["n_spect"]
 
     def close(self) -> None:
         if getattr(self, "_h", None):
-            _lib.ns2d_destroy(self._h)
+            if self._owned:
+                _lib.ns2d_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -253,8 +293,65 @@ class Solver:
         _check(_lib.ns2d_get_stream(self._h, ctypes.byref(s)))
         return s.value or 0
 
+    def set_kpass_variant(self, idx: int) -> None:
+        _check(_lib.ns2d_set_variant(self._h, PASS_COLUMN, int(idx)))
+
+    def set_rpass_variant(self, idx: int) -> None:
+        _check(_lib.ns2d_set_variant(self._h, PASS_ROW, int(idx)))
+
     def profile_steps(self, dt: float, nsteps: int):
         k, r, t = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         _check(_lib.ns2d_profile_steps(self._h, float(dt), int(nsteps), ctypes.byref(k), ctypes.byref(r),
                                        ctypes.byref(t)))
         return k.value, r.value, t.value
+
+
+class Ensemble:
+    """Many independent RK4CN solvers (any mix of grid sizes) stepped together:
+    members of one size share batched kernel launches and a two-step period of
+    all members replays as one hipGraph (ns2d_ens_* in the C ABI)."""
+
+    def __init__(self, cfgs):
+        lib = load_library()
+        self.cfgs = list(cfgs)
+        arr = (_Params * len(self.cfgs))(*[c._c() for c in self.cfgs])
+        h = ctypes.c_void_p()
+        _check(lib.ns2d_ens_create(arr, len(self.cfgs), ctypes.byref(h)))
+        self._h = h
+        self.members = []
+        for i, c in enumerate(self.cfgs):
+            mh = ctypes.c_void_p()
+            _check(lib.ns2d_ens_member(self._h, i, ctypes.byref(mh)))
+            self.members.append(Solver(c, _handle=mh))
+
+    def __len__(self):
+        return len(self.members)
+
+    def __getitem__(self, i) -> Solver:
+        return self.members[i]
+
+    def step(self, dts, nsteps: int) -> None:
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dts, dtype=np.float64), (len(self.members),)))
+        _check(_lib.ns2d_ens_step(self._h, _dptr(d), int(nsteps)))
+
+    def sync(self) -> None:
+        _check(_lib.ns2d_ens_sync(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            for m in self.members:
+                m._h = None
+            _lib.ns2d_ens_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

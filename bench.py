@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""Benchmark: RK4 (RK4-Crank-Nicolson) steps/s of the FP64 pseudospectral
+vorticity solver on one MI355X.
+
+Workload (the one the headline metric is quoted on): decaying turbulence on a
+1024^2 grid, FP64, random-phase initial w_hat with E(k) ~ k^4 exp(-(k/8)^2)
+normalised to E = 0.5, nu = 5e-5 (nu k^2 viscosity), Hou-Li filter in the
+reference's integer-division form, RK4CN, dt = 5e-4, with the reference's
+per-step max|w| blow-up check.  A "step" is one full RK4CN time step (four
+nonlinear right-hand sides).  The state is resident in HBM before timing
+starts.  The same step is also timed on a 4096^2 grid (reported under
+"extra").
+
+The roofline object covers the dominant kernel, timed live with HIP events
+recorded on the solver's own stream around every launch; its algorithmic
+bytes per launch are stated in ALG_BYTES below.  The CPU baseline is the
+numpy/scipy restatement of the reference algorithm (oracle/) on the host
+cores, timed on a bounded number of steps of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "2d-navier-stokes_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+NU, DT, K0 = 5e-5, 5e-4, 8.0
+
+
+def alg_bytes(nx: int, ny: int) -> dict:
+    """Algorithmic HBM bytes per launch (FP64 complex = 16 B per mode, M = Nx*(Ny/2+1)).
+
+    rpass: reads the four x-transformed fields G1..G4, writes N          -> 5*16*M
+    kpass: reads N and w_hat, reads/writes the RK accumulator, writes the four
+           G's (stage 1: acc write only; stage 4: acc read + w_hat write)
+           -> (112 + 128 + 128 + 128)/4 * M on average over a step
+    step:  4 rpass + 4 kpass = 816*M;  the two-pass model of the survey
+           (separate inverse and forward column passes) is 928*M.
+    """
+    M = nx * (ny // 2 + 1)
+    return {"rpass": 80.0 * M, "kpass": 124.0 * M, "step": 816.0 * M, "step_model": 928.0 * M}
+
+
+def make_ic(nx, ny, seed=1):
+    from ns2d import ics
+    return ics.random_phase(nx, ny, seed=seed, k0=K0, e0=0.5, filt="houli_ref")
+
+
+def make_solver(nx, ny):
+    import ns2d
+    cfg = ns2d.Config(Nx=nx, Ny=ny, nu=NU, hyper_flag=0, hyper_pow=1.0, alpha=0.0, drag_flag=0,
+                      drag_pow=0.0, filter="houli_ref", integrator="RK4CN", check_every=1)
+    return ns2d.Solver(cfg)
+
+
+def time_gpu(nx, ny, steps, warmup):
+    s = make_solver(nx, ny)
+    s.set_state(make_ic(nx, ny))
+    s.spectral_step(DT, warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.spectral_step(DT, steps)
+    s.sync()
+    t1 = time.perf_counter()
+    # live per-kernel timing (HIP events on the solver's stream)
+    kms, rms, tot = s.profile_steps(DT, max(5, min(steps, 50)))
+    m = s.system_measures()
+    s.close()
+    return (t1 - t0), kms, rms, m
+
+
+def cpu_baseline(nx, ny, budget_s=20.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ns2d_oracle as O
+    cores = max(1, min(16, os.cpu_count() or 1))
+    O.set_fft_workers(cores)
+    p = O.Params(nx, ny, nu=NU, hyper_flag=0, hyper_pow=1.0, filter="houli_ref")
+    st = O.State(make_ic(nx, ny), p)
+    O.step(st, DT, 1)  # warm-up (plans, page faults)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        O.step(st, DT, 1)
+        O.max_vort(st.w_hat, p)   # the reference's per-step SystemCheck
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or n >= 200:
+            break
+    return {"value": n / el, "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": f"{n} RK4CN steps of the {nx}x{ny} workload incl. per-step max|w| "
+                      f"(numpy/scipy.fft restatement, {cores} FFT threads), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-4096", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU (slab-decomposed) runs are not "
+              "part of it -- use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    nx = ny = args.n
+    el, kms, rms, meas = time_gpu(nx, ny, args.steps, args.warmup)
+    sps = args.steps / el
+    ab = alg_bytes(nx, ny)
+    if kms >= rms:
+        dom, dom_ms, dom_bytes = "kpass", kms, ab["kpass"]
+    else:
+        dom, dom_ms, dom_bytes = "rpass", rms, ab["rpass"]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    extra = {
+        "kpass_avg_ms": kms, "rpass_avg_ms": rms,
+        "step_alg_GBps": ab["step"] * sps / 1e9,
+        "step_model928_GBps": ab["step_model"] * sps / 1e9,
+        "energy_after": meas["energy"],
+    }
+    if not args.no_4096 and args.n != 4096:
+        el4, k4, r4, _ = time_gpu(4096, 4096, 20, 3)
+        s4 = 20 / el4
+        ab4 = alg_bytes(4096, 4096)
+        extra.update({"rk4_4096_steps_per_s": s4, "rk4_4096_ms_per_step": 1e3 / s4,
+                      "rk4_4096_kpass_avg_ms": k4, "rk4_4096_rpass_avg_ms": r4,
+                      "rk4_4096_step_alg_GBps": ab4["step"] * s4 / 1e9,
+                      "rk4_4096_step_model928_frac": ab4["step_model"] * s4 / 1e9 / HBM_PEAK_GBS})
+    out = {
+        "metric": "RK4 steps/s at 1024^2 FP64 (RK4CN, decaying turbulence)",
+        "value": sps,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 / sps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random-phase vorticity, E(k)~k^4 exp(-(k/8)^2), E=0.5)",
+        "config": {"workload": f"decaying turbulence {nx}x{ny} FP64, RK4CN, Hou-Li (integer-division "
+                               f"form), nu=5e-5, dt=5e-4, per-step max|w| check",
+                   "grid": [nx, ny], "integrator": "RK4CN", "filter": "houli_ref", "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(nx, ny, args.cpu_budget),
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -149,6 +149,34 @@ int ns2d_get_stream(ns2d_t* h, void** stream);
 int ns2d_profile_steps(ns2d_t* h, double dt, long nsteps, double* kpass_ms, double* rpass_ms,
                        double* total_ms);
 
+/* Tuning.  Each pass has several compiled shapes per transform length:
+ *   column pass (Nx): {columns per workgroup, rows per thread, threads,
+ *                      workgroups sharing a column group}
+ *   row pass (Ny):    {threads per row pair, points per thread, threads,
+ *                      row pairs per workgroup}
+ * The project's autotuner times them and picks one per grid; index 0 is the
+ * built-in default.  No counterpart in the reference (FFTW_MEASURE planning,
+ * solver.c:2883-2895, plays this role there). */
+#define NS2D_PASS_COLUMN 0
+#define NS2D_PASS_ROW 1
+int ns2d_variant_count(int pass, int n);
+int ns2d_variant_info(int pass, int n, int idx, int* shape4);
+int ns2d_set_variant(ns2d_t* h, int pass, int idx);
+
+/* Ensemble (no counterpart in the reference, which runs one realisation per
+ * process): n independent RK4CN solvers of possibly different grid sizes,
+ * stepped together.  Members of one size share batched kernel launches; a
+ * two-step period of all members is replayed as one hipGraph.  Member
+ * handles (ns2d_ens_member) accept every single-solver call (set_state,
+ * get_state, measures, ...) and must not be destroyed individually. */
+typedef struct ns2d_ensemble ns2d_ens_t;
+int ns2d_ens_create(const ns2d_params* members, int n, ns2d_ens_t** out);
+int ns2d_ens_member(ns2d_ens_t* e, int i, ns2d_t** h);
+/* nsteps RK4CN steps of every member, member i with step dt[i]. */
+int ns2d_ens_step(ns2d_ens_t* e, const double* dt, long nsteps);
+int ns2d_ens_sync(ns2d_ens_t* e);
+int ns2d_ens_destroy(ns2d_ens_t* e);
+
 const char* ns2d_last_error(void);
 
 #ifdef __cplusplus

@@ -78,6 +78,11 @@ KAPPA = 1.0                   # data_types.h:140
 
 _FFT_WORKERS = 1
 
+# element-wise libm functions: the reference calls the C library's exp/pow
+# (glibc, correctly rounded here); numpy's vectorised exp can differ by 1 ulp
+_libm_exp = np.vectorize(math.exp, otypes=[np.float64])
+_libm_pow = np.vectorize(math.pow, otypes=[np.float64])
+
 
 def set_fft_workers(n: int) -> None:
     """Threads used by the oracle's FFTs (only the CPU baseline sets >1)."""
@@ -165,11 +170,11 @@ def dealias_factor(p: Params) -> np.ndarray:
         hx, hy = p.Nx // 2, p.Ny // 2
         qx = (np.sign(kx) * (np.abs(kx) // hx)).astype(np.float64)  # C truncation
         qy = (np.sign(ky) * (np.abs(ky) // hy)).astype(np.float64)
-        return np.exp(-36.0 * np.power(np.sqrt(qx ** 2 + qy ** 2), 36.0))
+        return _libm_exp(-36.0 * _libm_pow(np.sqrt(qx ** 2 + qy ** 2), 36.0))
     if p.filter == "houli":
         qx = kx / (p.Nx / 2.0)
         qy = ky / (p.Ny / 2.0)
-        return np.exp(-36.0 * np.power(np.sqrt(qx ** 2 + qy ** 2), 36.0))
+        return _libm_exp(-36.0 * _libm_pow(np.sqrt(qx ** 2 + qy ** 2), 36.0))
     if p.filter == "none":
         return np.ones((p.Nx, p.Nyf))
     raise ValueError(p.filter)
@@ -252,7 +257,7 @@ def rhs_with_side_effect(arg: np.ndarray, state: np.ndarray, p: Params) -> np.nd
 # ---------------------------------------------------------------------------
 def _pw(k2: np.ndarray, e: float) -> np.ndarray:
     # the (0,0) mode is excluded by every caller (its update is forced to 0)
-    return np.power(np.where(k2 > 0, k2, 1.0), e)
+    return _libm_pow(np.where(k2 > 0, k2, 1.0), e)
 
 
 def visc_coeff(p: Params, hyper_flag: int, branch: str = "rk") -> np.ndarray:
@@ -417,14 +422,14 @@ def measures(w_hat: np.ndarray, p: Params, hyper_flag: int | None = None,
     hyp = hyper_flag == 1
     drag = p.drag_flag == 1
     if hyp and drag:
-        pre = p.nu * np.power(k2, p.hyper_pow) + p.alpha * np.power(k2s, p.drag_pow)
+        pre = p.nu * _libm_pow(k2, p.hyper_pow) + p.alpha * _libm_pow(k2s, p.drag_pow)
     elif (not hyp) and drag:
-        pre = p.nu * k2 + p.alpha * np.power(k2s, p.drag_pow)
+        pre = p.nu * k2 + p.alpha * _libm_pow(k2s, p.drag_pow)
     elif hyp and not drag:
-        pre = p.nu * np.power(k2, p.hyper_pow)
+        pre = p.nu * _libm_pow(k2, p.hyper_pow)
     else:
         pre = p.nu * k2
-    a2 = (w_hat.real ** 2 + w_hat.imag ** 2)
+    a2 = (w_hat.real ** 2 + w_hat.imag ** 2)   # cabs(w * conj(w)), sys_msr.c:191
     a2 = np.where(nz, a2, 0.0)
     u_z = 1j * ky * w_hat / k2s
     v_z = -1j * kx * w_hat / k2s

@@ -1,0 +1,61 @@
+"""CPU-side checks of the C ABI and the host package (no GPU calls)."""
+import os
+import re
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "ns2d_gpu.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(ns2d_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    import ns2d
+    lib = ns2d.load_library()
+    declared = header_functions()
+    assert len(declared) >= 20
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert sorted(ns2d.EXPORTS) == declared
+
+
+def test_variant_tables_cover_every_size():
+    import ns2d
+    for n in (8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096):
+        vs = ns2d.kpass_variants(n)
+        assert vs, n
+        for v in vs:
+            assert v["threads"] % v["cols"] == 0 and 4 % v["splits"] == 0
+            assert v["per_thread"] * v["threads"] == n * v["cols"]
+        rs = ns2d.rpass_variants(n)
+        assert rs, n
+        for v in rs:
+            assert v["per_thread"] * v["threads_per_pair"] == n
+            assert v["threads"] == v["threads_per_pair"] * v["pairs"]
+            assert v["threads_per_pair"] > 64 and v["pairs"] == 1 or 64 % v["threads_per_pair"] == 0
+
+
+def test_ics_match_oracle_generator():
+    """The product IC generator (ns2d.ics) and the oracle's produce identical w_hat."""
+    import ns2d_oracle as O
+    from ns2d import ics
+    for nx, ny, f in ((64, 64, "23"), (128, 32, "houli_ref"), (32, 64, "houli")):
+        a = O.ic_random_spectrum(O.Params(nx, ny, filter=f), seed=5, k0=6.0)
+        b = ics.random_phase(nx, ny, seed=5, k0=6.0, filt=f)
+        assert np.array_equal(a, b)
+
+
+def test_invalid_params_rejected_without_gpu():
+    """Argument validation happens before any device call."""
+    import ctypes
+
+    import ns2d
+    lib = ns2d.load_library()
+    p = ns2d.Config(Nx=100, Ny=64)._c()
+    h = ctypes.c_void_p()
+    rc = lib.ns2d_create(ctypes.byref(p), ctypes.byref(h))
+    assert rc == ns2d.NS2D_ERR_INVALID
+    assert b"powers of two" in lib.ns2d_last_error()

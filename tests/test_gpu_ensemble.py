@@ -1,0 +1,72 @@
+"""GPU: the batched ensemble path against independent single-solver runs and
+the oracle.  Members of different sizes, viscosities and step sizes share
+batched launches and a replayed hipGraph; each member must reproduce its own
+plain solve (same kernels, same arithmetic: bit-identical) and the oracle."""
+import numpy as np
+import pytest
+
+import ns2d_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def cfg_of(p):
+    import ns2d
+    return ns2d.Config(Nx=p.Nx, Ny=p.Ny, nu=p.nu, hyper_flag=p.hyper_flag, hyper_pow=p.hyper_pow,
+                       alpha=p.alpha, drag_flag=p.drag_flag, drag_pow=p.drag_pow, filter=p.filter,
+                       integrator="RK4CN")
+
+
+def test_ensemble_matches_plain_solves_and_oracle():
+    import ns2d
+    specs = [(64, 64, 1e-3, 1e-3), (128, 128, 2e-3, 7e-4), (64, 64, 5e-4, 1.5e-3),
+             (256, 128, 1e-3, 5e-4), (128, 128, 1e-3, 1e-3), (64, 64, 1e-3, 2e-3)]
+    params, w0s, dts = [], [], []
+    for i, (nx, ny, nu, dt) in enumerate(specs):
+        p = O.Params(nx, ny, nu=nu, hyper_flag=0, hyper_pow=1.0, filter="23")
+        params.append(p)
+        w0s.append(O.ic_random_spectrum(p, seed=100 + i, k0=4.0 + i))
+        dts.append(dt)
+    nsteps = 7    # odd: graph periods plus a single batched step
+    print("ensemble of", len(specs), flush=True)
+    with ns2d.Ensemble([cfg_of(p) for p in params]) as ens:
+        for i, w in enumerate(w0s):
+            ens[i].set_state(w)
+        ens.step(dts, 3)
+        ens.step(dts, nsteps - 3)
+        got = [ens[i].get_state() for i in range(len(ens))]
+        info = [ens[i].info() for i in range(len(ens))]
+    for i, p in enumerate(params):
+        assert info[i]["iters"] == nsteps
+        with ns2d.Solver(cfg_of(p)) as s:
+            s.set_state(w0s[i])
+            s.spectral_step(dts[i], nsteps)
+            plain = s.get_state()
+        assert np.array_equal(got[i], plain), i
+        st = O.State(w0s[i], p)
+        O.step(st, dts[i], nsteps)
+        assert np.linalg.norm(got[i] - st.w_hat) / np.linalg.norm(st.w_hat) < 1e-12
+
+
+def test_graph_replay_matches_direct_launches():
+    """hipGraph replay of step periods (default) equals direct launches."""
+    import os
+
+    import ns2d
+    p = O.Params(128, 128, nu=1e-3, hyper_flag=0, hyper_pow=1.0)
+    w0 = O.ic_random_spectrum(p, seed=3, k0=6.0)
+    with ns2d.Solver(cfg_of(p)) as s:
+        s.set_state(w0)
+        for _ in range(3):
+            s.spectral_step(1e-3, 5)
+            s.sync()
+        a = s.get_state()
+    os.environ["NS2D_GRAPH"] = "0"
+    try:
+        with ns2d.Solver(cfg_of(p)) as s:
+            s.set_state(w0)
+            s.spectral_step(1e-3, 15)
+            b = s.get_state()
+    finally:
+        del os.environ["NS2D_GRAPH"]
+    assert np.array_equal(a, b)

@@ -151,3 +151,41 @@ def test_measures_match_oracle(Nx, Ny, forcing, with_flux):
         for k in ("d_enst_dt_spect", "enst_diss_spect", "enst_flux_spect",
                   "d_enrg_dt_spect", "enrg_diss_spect", "enrg_flux_spect"):
             assert rel(got[k], ref[k]) < 1e-10, k
+
+
+def _variant_cases():
+    import ns2d
+    try:
+        ns2d.load_library()
+    except OSError:
+        return []
+    cases = []
+    for nx, ny in [(128, 32), (256, 32), (512, 32), (1024, 32), (2048, 16), (4096, 16)]:
+        for i, _ in enumerate(ns2d.kpass_variants(nx)):
+            cases.append((nx, ny, i, 0))
+    for ny, nx in [(32, 16), (64, 16), (128, 16), (256, 16), (512, 16), (1024, 16), (2048, 16), (4096, 16)]:
+        for j, _ in enumerate(ns2d.rpass_variants(ny)):
+            cases.append((nx, ny, 0, j))
+    return cases
+
+
+@pytest.mark.parametrize("Nx,Ny,var,rvar", _variant_cases())
+@pytest.mark.parametrize("integ", ["RK4CN", "AB4"])
+def test_pass_variants_match_oracle(Nx, Ny, var, rvar, integ):
+    """Every compiled column-pass and row-pass shape gives the oracle's trajectory."""
+    print(f"variant {Nx}x{Ny} k#{var} r#{rvar} {integ}", flush=True)
+    p = O.Params(Nx, Ny, nu=2e-3, hyper_flag=1, hyper_pow=2.0, alpha=0.05, drag_flag=1, drag_pow=0.0,
+                 filter="23", integrator=integ, forcing="KOLM", force_k=2, force_scale=0.5)
+    w0 = O.ic_random_spectrum(p, seed=Nx + var, k0=3.0)
+    w0[Nx - 1, 0] += 0.2 * np.abs(w0).max()
+    st = O.State(w0, p)
+    O.step(st, 1e-3, 6)
+    with solver(p) as s:
+        s.set_kpass_variant(var)
+        s.set_rpass_variant(rvar)
+        s.set_state(w0)
+        s.spectral_step(1e-3, 6)
+        got = s.get_state()
+        K = s.nonlinear_rhs(w0)
+    assert rel(got, st.w_hat) < 1e-12
+    assert rel(K, O.nonlinear_rhs(w0, p)) < 1e-12
