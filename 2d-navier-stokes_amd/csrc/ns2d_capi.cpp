@@ -79,6 +79,12 @@ struct ns2d_handle {
     int wcur, cur, ring, kvar, rvar, check, period;
     const double* lam;
   } gkey{};
+  // adjoint buffers (allocated on first use)
+  double2* tape = nullptr;     // [slot][4 fields][M], slot = 4*step + stage
+  long tape_slots = 0;
+  double2* ckpt = nullptr;     // [segment][M]
+  long ckpt_n = 0;
+  double2 *lamp[2] = {nullptr, nullptr}, *lacc = nullptr, *lbuf = nullptr, *bbuf[4] = {nullptr, nullptr, nullptr, nullptr};
   // profiling
   bool profiling = false;
   std::vector<hipEvent_t> ev;
@@ -535,6 +541,10 @@ int ns2d_destroy(ns2d_t* h) {
     if (b) (void)hipFree(b);
   for (double* t : h->lam_tab)
     if (t) (void)hipFree(t);
+  double2* adj[] = {h->tape, h->ckpt, h->lamp[0], h->lamp[1], h->lacc, h->lbuf, h->bbuf[0], h->bbuf[1], h->bbuf[2],
+                    h->bbuf[3]};
+  for (double2* b : adj)
+    if (b) (void)hipFree(b);
   if (h->filt_tab) (void)hipFree(h->filt_tab);
   if (h->stat) (void)hipFree(h->stat);
   if (h->meas) (void)hipFree(h->meas);
@@ -1142,6 +1152,216 @@ int ns2d_ens_destroy(ns2d_ens_t* e) {
   for (ns2d_handle* h : e->m) ns2d_destroy(h);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
+  return NS2D_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of the RK4CN map (no counterpart in the reference).
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+int adj_alloc(ns2d_handle* h, long slots, long nckpt) {
+  const size_t fb = h->M * sizeof(double2);
+  if (h->tape_slots < slots) {
+    if (h->tape) (void)hipFree(h->tape);
+    h->tape = nullptr;
+    h->tape_slots = 0;
+    if (hipMalloc((void**)&h->tape, fb * 4 * (size_t)slots) != hipSuccess)
+      return fail(NS2D_ERR_HIP, "adjoint tape of %ld stage slots (%.1f GB) does not fit", slots,
+                  fb * 4.0 * slots / 1e9);
+    h->tape_slots = slots;
+  }
+  if (h->ckpt_n < nckpt) {
+    if (h->ckpt) (void)hipFree(h->ckpt);
+    h->ckpt = nullptr;
+    h->ckpt_n = 0;
+    if (hipMalloc((void**)&h->ckpt, fb * (size_t)nckpt) != hipSuccess)
+      return fail(NS2D_ERR_HIP, "adjoint checkpoints do not fit");
+    h->ckpt_n = nckpt;
+  }
+  double2** bufs[] = {&h->lamp[0], &h->lamp[1], &h->lacc, &h->lbuf, &h->bbuf[0], &h->bbuf[1], &h->bbuf[2], &h->bbuf[3]};
+  for (double2** b : bufs) {
+    if (*b) continue;
+    if (hipMalloc((void**)b, fb) != hipSuccess) return fail(NS2D_ERR_HIP, "adjoint buffer allocation failed");
+    HIPCHK(hipMemsetAsync(*b, 0, fb, h->stream));
+  }
+  return NS2D_OK;
+}
+
+double2* tape_slot(ns2d_handle* h, long slot, int f) { return h->tape + ((size_t)slot * 4 + f) * h->M; }
+
+// RK4CN steps [0, cnt) from the current state with every stage's G1..G4 kept
+// in tape slots 4*j + s (slot 4*cnt holds the final state's).
+int taped_forward(ns2d_handle* h, double dt, long cnt) {
+  KPassArgs a0 = kbase(h);
+  a0.mode = KP_INV;
+  a0.epi = EPI_FROM_STATE;
+  a0.conj_dst = h->conj[h->cur];
+  for (int f = 0; f < 4; ++f) a0.g[f] = tape_slot(h, 0, f);
+  int rc = run_kpass(h, a0);
+  if (rc) return rc;
+  for (long j = 0; j < cnt; ++j) {
+    for (int st = 0; st < 4; ++st) {
+      RPassArgs ra;
+      KPassArgs ka;
+      rc = rk4cn_stage_args(h, dt, false, st, &ra, &ka);
+      if (rc) return rc;
+      for (int f = 0; f < 4; ++f) {
+        ra.g[f] = tape_slot(h, 4 * j + st, f);
+        ka.g[f] = tape_slot(h, 4 * j + st + 1, f);
+      }
+      rc = run_rpass(h, ra);
+      if (rc) return rc;
+      rc = run_kpass(h, ka);
+      if (rc) return rc;
+    }
+    after_rk_update(h);
+    h->iters += 1;
+    h->wcur ^= 1;
+  }
+  h->inv_valid = false;
+  return NS2D_OK;
+}
+
+// dJ/dw_0 for J depending on w_n = Phi^n(w_0) through dJ/dw_n (in lamp[0]);
+// w_0 in w[wcur].  Result in lamp[h->cur ... ] returned through *res.
+int vjp_impl(ns2d_handle* h, double dt, long n, long ckpt_every, double2** res) {
+  if (h->p.quirks & NS2D_QUIRK_CONJ_GLOBAL)
+    return fail(NS2D_ERR_INVALID, "the adjoint needs the ForceConjugacy quirk off (quirks & NS2D_QUIRK_CONJ_GLOBAL)");
+  if (h->p.integrator != NS2D_INT_RK4CN) return fail(NS2D_ERR_INVALID, "the adjoint covers the RK4CN integrator");
+  const long C = (ckpt_every <= 0 || ckpt_every > n) ? n : ckpt_every;
+  const long S = (n + C - 1) / C;
+  int rc = adj_alloc(h, 4 * C + 1, S);
+  if (rc) return rc;
+  const size_t fb = h->M * sizeof(double2);
+  // D_fac tables of step 0 and of the later steps (the Q3 flag update)
+  h->hyper_eff = h->p.hyper_flag;
+  const double* lam0 = lam_table(h, visc_kind_rk(h));
+  after_rk_update(h);
+  const double* lam1 = lam_table(h, visc_kind_rk(h));
+  h->hyper_eff = h->p.hyper_flag;
+  if (!lam0 || !lam1) return fail(NS2D_ERR_HIP, "coefficient table allocation failed");
+  auto lam_of = [&](long step) { return step == 0 ? lam0 : lam1; };
+  // forward sweep: checkpoints, the last segment taped
+  h->iters = 0;
+  for (long seg = 0; seg < S; ++seg) {
+    HIPCHK(hipMemcpyAsync(h->ckpt + (size_t)seg * h->M, h->w[h->wcur], fb, hipMemcpyDeviceToDevice, h->stream));
+    const long cnt = (seg == S - 1) ? n - seg * C : C;
+    if (seg == S - 1) {
+      rc = taped_forward(h, dt, cnt);
+    } else {
+      rc = prepare_inv(h);
+      for (long j = 0; j < cnt && rc == NS2D_OK; ++j) rc = rk4cn_step(h, dt, false);
+    }
+    if (rc) return rc;
+    h->inv_valid = false;
+  }
+  // backward sweep
+  int lc = 0;   // lamp[lc] holds dJ/dw at the current step boundary
+  bool started = false;
+  for (long seg = S - 1; seg >= 0; --seg) {
+    const long cnt = (seg == S - 1) ? n - seg * C : C;
+    if (seg != S - 1) {
+      HIPCHK(hipMemcpyAsync(h->w[h->wcur], h->ckpt + (size_t)seg * h->M, fb, hipMemcpyDeviceToDevice, h->stream));
+      h->iters = seg * C;
+      h->hyper_eff = h->p.hyper_flag;
+      if (h->iters > 0) after_rk_update(h);
+      rc = taped_forward(h, dt, cnt);
+      if (rc) return rc;
+    }
+    for (long j = cnt - 1; j >= 0; --j) {
+      const long ng = seg * C + j;
+      KAdjArgs ka{};
+      ka.tw = h->tw;
+      ka.nyf = h->nyf;
+      ka.ld = h->ld;
+      ka.dt = dt;
+      ka.ph = h->ph;
+      ka.lacc = h->lacc;
+      ka.lout = h->lbuf;
+      for (int f = 0; f < 4; ++f) ka.bin[f] = h->bbuf[f];
+      ka.lam = lam_of(ng);
+      if (!started) {
+        KAdjArgs k0 = ka;
+        k0.mode = KA_INV;
+        k0.epi = ADJ_START;
+        k0.lamp_in = h->lamp[lc];
+        hipError_t e = launch_kpass_adj(h->p.nx, k0, h->stream);
+        if (e != hipSuccess) return fail(NS2D_ERR_HIP, "adjoint kpass: %s", hipGetErrorString(e));
+        started = true;
+      }
+      const int epis[4] = {ADJ_S1, ADJ_S2, ADJ_S3, ADJ_S4};
+      for (int st = 3; st >= 0; --st) {
+        RAdjArgs ra{};
+        ra.lin = h->lbuf;
+        for (int f = 0; f < 4; ++f) {
+          ra.g[f] = tape_slot(h, 4 * j + st, f);
+          ra.bout[f] = h->bbuf[f];
+        }
+        ra.tw = h->tw;
+        ra.nx = h->p.nx;
+        ra.ld = h->ld;
+        hipError_t e = launch_rpass_adj(h->p.nx, h->p.ny, ra, h->stream);
+        if (e != hipSuccess) return fail(NS2D_ERR_HIP, "adjoint rpass: %s", hipGetErrorString(e));
+        KAdjArgs kk = ka;
+        kk.mode = KA_FWD | ((st > 0 || ng > 0) ? KA_INV : 0);
+        kk.epi = epis[st];
+        kk.lamp_in = h->lamp[lc];
+        if (st == 0) {
+          kk.lamp_out = h->lamp[lc ^ 1];
+          kk.lam_prev = ng > 0 ? lam_of(ng - 1) : nullptr;
+        }
+        e = launch_kpass_adj(h->p.nx, kk, h->stream);
+        if (e != hipSuccess) return fail(NS2D_ERR_HIP, "adjoint kpass: %s", hipGetErrorString(e));
+      }
+      lc ^= 1;
+    }
+  }
+  *res = h->lamp[lc];
+  h->inv_valid = false;
+  return NS2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ns2d_vjp_device(ns2d_t* h, const void* d_w0, double dt, long nsteps, const void* d_lam_T, void* d_grad_w0,
+                    long ckpt_every) {
+  if (!h || !d_w0 || !d_lam_T || !d_grad_w0) return fail(NS2D_ERR_INVALID, "null argument");
+  if (nsteps < 1 || !(dt > 0.0)) return fail(NS2D_ERR_INVALID, "need nsteps >= 1 and dt > 0");
+  HIPCHK(hipSetDevice(h->p.device));
+  int rc = adj_alloc(h, 1, 1);
+  if (rc) return rc;
+  const size_t rowb = h->nyf * sizeof(double2), pitch = h->ld * sizeof(double2);
+  HIPCHK(hipMemcpy2DAsync(h->w[h->wcur], pitch, d_w0, rowb, rowb, h->p.nx, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(hipMemcpy2DAsync(h->lamp[0], pitch, d_lam_T, rowb, rowb, h->p.nx, hipMemcpyDeviceToDevice, h->stream));
+  double2* res = nullptr;
+  rc = vjp_impl(h, dt, nsteps, ckpt_every, &res);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy2DAsync(d_grad_w0, rowb, res, pitch, rowb, h->p.nx, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return NS2D_OK;
+}
+
+int ns2d_vjp(ns2d_t* h, const double* w0, double dt, long nsteps, const double* lam_T, double* grad_w0,
+             long ckpt_every) {
+  if (!h || !w0 || !lam_T || !grad_w0) return fail(NS2D_ERR_INVALID, "null argument");
+  if (nsteps < 1 || !(dt > 0.0)) return fail(NS2D_ERR_INVALID, "need nsteps >= 1 and dt > 0");
+  HIPCHK(hipSetDevice(h->p.device));
+  int rc = adj_alloc(h, 1, 1);
+  if (rc) return rc;
+  const size_t rowb = h->nyf * sizeof(double2), pitch = h->ld * sizeof(double2);
+  HIPCHK(hipMemcpy2DAsync(h->w[h->wcur], pitch, w0, rowb, rowb, h->p.nx, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpy2DAsync(h->lamp[0], pitch, lam_T, rowb, rowb, h->p.nx, hipMemcpyHostToDevice, h->stream));
+  double2* res = nullptr;
+  rc = vjp_impl(h, dt, nsteps, ckpt_every, &res);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy2DAsync(grad_w0, rowb, res, pitch, rowb, h->p.nx, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
   return NS2D_OK;
 }
 

@@ -84,6 +84,32 @@ struct RPassArgs {
   double wscale;
 };
 
+// adjoint kpass modes and epilogues
+enum { KA_FWD = 1, KA_INV = 2 };
+enum { ADJ_START = 0, ADJ_S4 = 1, ADJ_S3 = 2, ADJ_S2 = 3, ADJ_S1 = 4 };
+
+struct KAdjArgs {
+  const double2* bin[4];   // row-pass adjoint outputs B1..B4
+  double2* lout;           // x-inverse of Wh .* M .* l_K for the next stage
+  const double2* lamp_in;  // dJ/dw_{n+1}
+  double2* lamp_out;       // dJ/dw_n (ADJ_S1)
+  double2* lacc;           // running dJ/dw_n
+  const double* lam;       // D_fac/dt table of this step
+  const double* lam_prev;  // ... of the previous step (ADJ_S1 with KA_INV)
+  const double2* tw;
+  int nyf, ld, mode, epi;
+  double dt;
+  PhysArgs ph;
+};
+
+struct RAdjArgs {
+  const double2* lin;      // x-transformed r2c^T field
+  const double2* g[4];     // the stage's x-transformed G1..G4 (forward tape)
+  double2* bout[4];
+  const double2* tw;
+  int nx, ld;
+};
+
 struct MeasArgs {
   const double2* w_hat;
   const double2* rhs;
@@ -115,5 +141,7 @@ int rpass_variant_info(int ny, int idx, int* TLR, int* E, int* T, int* PAIRS);
 hipError_t launch_r2c_rows(int nx, int ny, const double* in, double2* out, const double2* tw, int ld,
                            hipStream_t s);
 hipError_t launch_measures(int nbins, const MeasArgs& a, hipStream_t s);
+hipError_t launch_kpass_adj(int nx, const KAdjArgs& a, hipStream_t s);
+hipError_t launch_rpass_adj(int nx, int ny, const RAdjArgs& a, hipStream_t s);
 
 }  // namespace ns2d

@@ -452,6 +452,232 @@ __global__ __launch_bounds__(TLR * PAIRS) void rpass_batched(const RPassArgs* __
 }
 
 // ---------------------------------------------------------------------------
+// Discrete adjoint (vector-Jacobian product) of the RK4CN step.
+//
+// With K = N(S) = M .* r2c(u w_x + v w_y), u = c2r(U S), v = c2r(V S),
+// w_x = c2r(X S), w_y = c2r(Y S) (U = i ky/|k|^2, V = -i kx/|k|^2, X = i kx,
+// Y = i ky, M = dealias * (1/(NxNy))^2, zero at k = 0) and gradients packed as
+// g = dJ/dRe + i dJ/dIm, the transposes are
+//   r2c^T(l) = c2r(Wh .* l),   Wh = 1/2 on interior ky columns, 1 on ky = 0, Ny/2
+//   c2r^T(f) = W2 .* r2c(f),   W2 = 2 on interior ky columns, 1 on ky = 0, Ny/2
+// so, with l_n = r2c^T(M .* l_K):
+//   N'(S)^T l_K = conj(U) W2 r2c(l_n w_x) + conj(X) W2 r2c(l_n u)
+//               + conj(V) W2 r2c(l_n w_y) + conj(Y) W2 r2c(l_n v).
+// rpass_adj forms the four products and their y transforms (the ky parts of
+// the multipliers and W2 applied); kpass_adj does the four x transforms, the
+// kx parts, the RK4CN adjoint recursion and r2c^T of the next stage.
+// ---------------------------------------------------------------------------
+template <int NY, int TLR, int PAIRS>
+__device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk) {
+  constexpr int H = NY / 2;
+  constexpr int E = NY / TLR;
+  constexpr int LS = line_stride(NY);
+  constexpr bool BLOCK = TLR > 64;
+  static_assert(E >= 2 && (BLOCK ? PAIRS == 1 : 64 % TLR == 0), "bad rpass_adj shape");
+  __shared__ double2 lds[PAIRS * LS];
+
+  const int tid = threadIdx.x;
+  const int q = tid / TLR, l = tid % TLR;
+  const int x0 = 2 * (blk * PAIRS + q);
+  const bool valid = x0 < a.nx;
+  if (!BLOCK && !valid) return;
+  const int ld = a.ld;
+  const double2* __restrict__ tw = a.tw;
+  double2* buf = lds + q * LS;
+
+  // full-length spectrum of A + iB from the half spectra of two real fields
+  auto build = [&](double2* z, const double2* fa, const double2* fb, int mula, int mulb, int lo) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int ky = lo + TLR * e;
+      const bool mirror = ky > H;
+      const int kk = mirror ? NY - ky : ky;
+      const double fk = (double)kk;
+      double2 A = fa[kk], B = fb[kk];
+      if (mula) A = make_double2(-fk * A.y, fk * A.x);   // times i ky
+      if (mulb) B = make_double2(-fk * B.y, fk * B.x);
+      if (kk == 0 || kk == H) { A.y = 0.0; B.y = 0.0; }
+      z[e] = mirror ? make_double2(A.x + B.y, B.x - A.y) : make_double2(A.x - B.y, A.y + B.x);
+      if ((e & 3) == 3) asm volatile("" ::: "memory");
+    }
+  };
+  // r2c of the real and imaginary parts of a transformed packed row:
+  // P[k] = (Z[k] + conj Z[-k])/2, Q[k] = (Z[k] - conj Z[-k])/(2i); writes
+  // W2 (mul ? -i k : 1) P to pout and W2 (mulq ? -i k : 1) Q to qout
+  auto split_store = [&](double2* z, double2* pout, double2* qout, int mulp, int mulq, int lo) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) buf[padx(lo + TLR * e)] = z[e];
+    xsync<BLOCK>();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = lo + TLR * e;
+      if (k <= H) {
+        const double2 zp = z[e];
+        const double2 zm = cconj(buf[padx((NY - k) & (NY - 1))]);
+        double2 P = make_double2(0.5 * (zp.x + zm.x), 0.5 * (zp.y + zm.y));
+        double2 Q = make_double2(0.5 * (zp.y - zm.y), -0.5 * (zp.x - zm.x));
+        const double w2 = (k == 0 || k == H) ? 1.0 : 2.0;
+        const double fk = (double)k;
+        if (mulp) P = make_double2(fk * P.y, -fk * P.x);   // times -i k
+        if (mulq) Q = make_double2(fk * Q.y, -fk * Q.x);
+        pout[k] = cscale(w2, P);
+        qout[k] = cscale(w2, Q);
+      }
+    }
+    xsync<BLOCK>();
+  };
+
+  // l_n of rows x0 and x0+1 from the x-transformed r2c^T field L
+  double2 zl[E];
+  build(zl, a.lin + (size_t)x0 * ld, a.lin + (size_t)(x0 + 1) * ld, 0, 0, opaque(l));
+  fft_reg<NY, TLR, +1, BLOCK>(zl, buf, tw, opaque(l));
+#pragma unroll 1
+  for (int r = 0; r < 2; ++r) {
+    const int x = opaque(x0 + r);
+    const size_t row = (size_t)x * ld;
+    double2 z1[E], z2[E];
+    build(z1, a.g[0] + row, a.g[1] + row, 1, 0, opaque(l));   // u + i v
+    fft_reg<NY, TLR, +1, BLOCK>(z1, buf, tw, opaque(l));
+    build(z2, a.g[2] + row, a.g[3] + row, 0, 1, opaque(l));   // w_x + i w_y
+    fft_reg<NY, TLR, +1, BLOCK>(z2, buf, tw, opaque(l));
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const double ln = (r == 0) ? zl[e].x : zl[e].y;
+      const double2 u = z1[e], wv = z2[e];
+      z1[e] = make_double2(ln * wv.x, ln * u.x);   // l_n w_x + i l_n u
+      z2[e] = make_double2(ln * wv.y, ln * u.y);   // l_n w_y + i l_n v
+    }
+    fft_reg<NY, TLR, -1, BLOCK>(z1, buf, tw, opaque(l));
+    split_store(z1, a.bout[0] + row, a.bout[1] + row, 1, 0, opaque(l));
+    fft_reg<NY, TLR, -1, BLOCK>(z2, buf, tw, opaque(l));
+    split_store(z2, a.bout[2] + row, a.bout[3] + row, 0, 1, opaque(l));
+  }
+}
+
+template <int NY, int TLR, int PAIRS>
+__global__ __launch_bounds__(TLR * PAIRS) void rpass_adj_kernel(RAdjArgs a) {
+  rpass_adj_body<NY, TLR, PAIRS>(a, (int)blockIdx.x);
+}
+
+template <int L, int C, int T>
+__global__ __launch_bounds__(T) void kpass_adj_kernel(KAdjArgs a) {
+  constexpr int TL = T / C;
+  constexpr int E = L / TL;
+  constexpr int LS = line_stride(L);
+  constexpr bool BLOCK = T > 64;
+  static_assert(T % C == 0 && L % TL == 0 && E >= 2, "bad kpass_adj shape");
+  __shared__ double2 lds[C * LS];
+
+  const int tid = threadIdx.x;
+  const int g = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int ky0 = g * C;
+  const int c = tid % C, l = tid / C;
+  const int ky = ky0 + c;
+  const int nyf = a.nyf, ld = a.ld, nx = L;
+  const bool colv = ky < nyf;
+  const double2* __restrict__ tw = a.tw;
+  const PhysArgs& ph = a.ph;
+  double2* buf = lds + c * LS;
+  const double h = a.dt;
+
+  double2 LSacc[E];   // N'(S)^T l_K of the stage just finished
+#pragma unroll
+  for (int e = 0; e < E; ++e) LSacc[e] = make_double2(0.0, 0.0);
+  if (a.mode & KA_FWD) {
+#pragma unroll 1
+    for (int f = 0; f < 4; ++f) {
+      const int l_o = opaque(l), ky_o = opaque(ky);
+      double2 F[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        F[e] = colv ? a.bin[f][(size_t)(l_o + TL * e) * ld + ky_o] : make_double2(0.0, 0.0);
+        if ((e & 3) == 3) asm volatile("" ::: "memory");
+      }
+      fft_reg<L, TL, -1, BLOCK>(F, buf, tw, l_o);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int kx = kx_of_row(l_o + TL * e, nx);
+        const int k2i = kx * kx + ky_o * ky_o;
+        const double inv = k2i ? 1.0 / (double)k2i : 0.0;
+        const double2 v = F[e];
+        double2 t;
+        if (f == 0) t = cscale(inv, v);                                   // conj(U): 1/|k|^2 (-i ky in B1)
+        else if (f == 1) t = make_double2(kx * v.y, -kx * v.x);          // conj(X) = -i kx
+        else if (f == 2) t = make_double2(-(kx * inv) * v.y, (kx * inv) * v.x);  // conj(V) = i kx/|k|^2
+        else t = v;                                                       // conj(Y): -i ky in B4
+        LSacc[e] = cadd(LSacc[e], t);
+      }
+    }
+  }
+
+  // RK4CN adjoint recursion (backwards through solver.c:690-853)
+  double2 LK[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    LK[e] = make_double2(0.0, 0.0);
+    if (!colv) continue;
+    const int kx = kx_of_row(l + TL * e, nx);
+    const size_t idx = (size_t)(l + TL * e) * ld + ky;
+    const bool zero = (kx == 0 && ky == 0);
+    const double D = h * a.lam[idx];
+    const double A = (2.0 - D) / (2.0 + D), B = 2.0 * h / (2.0 + D);
+    double2 lp = zero ? make_double2(0.0, 0.0) : a.lamp_in[idx];   // dJ/dw_{n+1}
+    switch (a.epi) {
+      case ADJ_START:
+        a.lacc[idx] = cscale(A, lp);
+        LK[e] = cscale(B / 6.0, lp);
+        break;
+      case ADJ_S4:
+        a.lacc[idx] = cadd(a.lacc[idx], LSacc[e]);
+        LK[e] = cadd(cscale(B / 3.0, lp), cscale(h, LSacc[e]));
+        break;
+      case ADJ_S3:
+        a.lacc[idx] = cadd(a.lacc[idx], LSacc[e]);
+        LK[e] = cadd(cscale(B / 3.0, lp), cscale(0.5 * h, LSacc[e]));
+        break;
+      case ADJ_S2:
+        a.lacc[idx] = cadd(a.lacc[idx], LSacc[e]);
+        LK[e] = cadd(cscale(B / 6.0, lp), cscale(0.5 * h, LSacc[e]));
+        break;
+      case ADJ_S1: {
+        // dJ/dw_n; with KA_INV it starts the adjoint of the previous step
+        const double2 lw = cadd(a.lacc[idx], LSacc[e]);
+        a.lamp_out[idx] = lw;
+        if (a.mode & KA_INV) {
+          const double Dp = h * a.lam_prev[idx];
+          const double Ap = (2.0 - Dp) / (2.0 + Dp), Bp = 2.0 * h / (2.0 + Dp);
+          const double2 lq = zero ? make_double2(0.0, 0.0) : lw;
+          a.lacc[idx] = cscale(Ap, lq);
+          LK[e] = cscale(Bp / 6.0, lq);
+        }
+        break;
+      }
+      default: break;
+    }
+    if ((e & 3) == 3) asm volatile("" ::: "memory");
+  }
+  if (!(a.mode & KA_INV)) return;
+
+  // l_n transform of the next stage: x-inverse of Wh .* M .* l_K
+  {
+    const int l_o = opaque(l), ky_o = opaque(ky);
+    const double wh = (ky_o == 0 || ky_o == ph.ny / 2) ? 1.0 : 0.5;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int kx = kx_of_row(l_o + TL * e, nx);
+      double m = 0.0;
+      if (colv && !(kx == 0 && ky_o == 0)) m = wh * ph.norm2 * dealias_factor(ph, kx, ky_o);
+      LK[e] = cscale(m, LK[e]);
+    }
+    fft_reg<L, TL, +1, BLOCK>(LK, buf, tw, l_o);
+    if (colv) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) a.lout[(size_t)(l_o + TL * e) * ld + ky_o] = LK[e];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // forward transform of a real field (ICs): r2c along y per row
 // ---------------------------------------------------------------------------
 template <int NY, int T>
@@ -808,6 +1034,55 @@ hipError_t launch_r2c_rows(int nx, int ny, const double* in, double2* out, const
 hipError_t launch_measures(int nbins, const MeasArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((measures_kernel<256>), dim3(nbins), dim3(256), 0, s, a);
   return hipGetLastError();
+}
+
+// adjoint pass shapes (one per transform length): the default forward shapes
+template <int NY>
+static hipError_t launch_rpass_adj_t(int nx, const RAdjArgs& a, hipStream_t s) {
+  constexpr int E = NY >= 16 ? 16 : NY;
+  constexpr int TLR = NY / E;
+  constexpr int PAIRS = TLR >= 64 ? 1 : 64 / TLR;
+  hipLaunchKernelGGL((rpass_adj_kernel<NY, TLR, PAIRS>), dim3((nx / 2 + PAIRS - 1) / PAIRS), dim3(TLR * PAIRS), 0,
+                     s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_rpass_adj(int nx, int ny, const RAdjArgs& a, hipStream_t s) {
+  switch (ny) {
+    case 8: return launch_rpass_adj_t<8>(nx, a, s);
+    case 16: return launch_rpass_adj_t<16>(nx, a, s);
+    case 32: return launch_rpass_adj_t<32>(nx, a, s);
+    case 64: return launch_rpass_adj_t<64>(nx, a, s);
+    case 128: return launch_rpass_adj_t<128>(nx, a, s);
+    case 256: return launch_rpass_adj_t<256>(nx, a, s);
+    case 512: return launch_rpass_adj_t<512>(nx, a, s);
+    case 1024: return launch_rpass_adj_t<1024>(nx, a, s);
+    case 2048: return launch_rpass_adj_t<2048>(nx, a, s);
+    case 4096: return launch_rpass_adj_t<4096>(nx, a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int L, int C, int T>
+static hipError_t launch_kpass_adj_t(const KAdjArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((kpass_adj_kernel<L, C, T>), dim3((a.nyf + C - 1) / C), dim3(T), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_kpass_adj(int nx, const KAdjArgs& a, hipStream_t s) {
+  switch (nx) {
+    case 8: return launch_kpass_adj_t<8, 16, 64>(a, s);
+    case 16: return launch_kpass_adj_t<16, 16, 64>(a, s);
+    case 32: return launch_kpass_adj_t<32, 16, 64>(a, s);
+    case 64: return launch_kpass_adj_t<64, 16, 64>(a, s);
+    case 128: return launch_kpass_adj_t<128, 16, 128>(a, s);
+    case 256: return launch_kpass_adj_t<256, 8, 128>(a, s);
+    case 512: return launch_kpass_adj_t<512, 4, 128>(a, s);
+    case 1024: return launch_kpass_adj_t<1024, 2, 128>(a, s);
+    case 2048: return launch_kpass_adj_t<2048, 2, 256>(a, s);
+    case 4096: return launch_kpass_adj_t<4096, 1, 256>(a, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace ns2d

@@ -65,7 +65,7 @@ EXPORTS = ["ns2d_params_default", "ns2d_create", "ns2d_destroy", "ns2d_set_state
            "ns2d_max_vort", "ns2d_real_vort", "ns2d_r2c", "ns2d_get_info", "ns2d_sync",
            "ns2d_get_stream", "ns2d_profile_steps", "ns2d_variant_count", "ns2d_variant_info",
            "ns2d_set_variant", "ns2d_ens_create", "ns2d_ens_member", "ns2d_ens_step", "ns2d_ens_sync",
-           "ns2d_ens_destroy", "ns2d_last_error"]
+           "ns2d_ens_destroy", "ns2d_vjp", "ns2d_vjp_device", "ns2d_last_error"]
 
 _lib = None
 
@@ -106,6 +106,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ns2d_ens_step": ([vp, dp, l], i),
         "ns2d_ens_sync": ([vp], i),
         "ns2d_ens_destroy": ([vp], i),
+        "ns2d_vjp": ([vp, dp, d, l, dp, dp, l], i),
+        "ns2d_vjp_device": ([vp, vp, d, l, vp, vp, l], i),
         "ns2d_last_error": ([], ctypes.c_char_p),
     }
     for name, (argt, rest) in sig.items():
@@ -292,6 +294,20 @@ class Solver:
         s = ctypes.c_void_p()
         _check(_lib.ns2d_get_stream(self._h, ctypes.byref(s)))
         return s.value or 0
+
+    def vjp(self, w0: np.ndarray, dt: float, nsteps: int, lam_T: np.ndarray, ckpt_every: int = 0) -> np.ndarray:
+        """dJ/dw_0 of nsteps RK4CN steps from w_0 given dJ/dw_n (packed dJ/dRe + i dJ/dIm)."""
+        w = np.ascontiguousarray(w0, dtype=np.complex128)
+        lt = np.ascontiguousarray(lam_T, dtype=np.complex128)
+        out = np.empty_like(w)
+        _check(_lib.ns2d_vjp(self._h, _dptr(w.view(np.float64)), float(dt), int(nsteps),
+                             _dptr(lt.view(np.float64)), _dptr(out.view(np.float64)), int(ckpt_every)))
+        return out
+
+    def vjp_device(self, w0_ptr: int, dt: float, nsteps: int, lam_T_ptr: int, grad_ptr: int,
+                   ckpt_every: int = 0) -> None:
+        _check(_lib.ns2d_vjp_device(self._h, ctypes.c_void_p(w0_ptr), float(dt), int(nsteps),
+                                    ctypes.c_void_p(lam_T_ptr), ctypes.c_void_p(grad_ptr), int(ckpt_every)))
 
     def set_kpass_variant(self, idx: int) -> None:
         _check(_lib.ns2d_set_variant(self._h, PASS_COLUMN, int(idx)))

@@ -177,6 +177,19 @@ int ns2d_ens_step(ns2d_ens_t* e, const double* dt, long nsteps);
 int ns2d_ens_sync(ns2d_ens_t* e);
 int ns2d_ens_destroy(ns2d_ens_t* e);
 
+/* Discrete adjoint (no counterpart in the reference): the vector-Jacobian
+ * product of nsteps RK4CN steps w_n = Phi^n(w_0).  Given lam_T = dJ/dw_n it
+ * returns grad_w0 = dJ/dw_0, both packed as dJ/dRe + i dJ/dIm in the
+ * [Nx][Ny/2+1] layout.  The forward map is re-run from w_0 with checkpoints
+ * every ckpt_every steps (<= 0: one segment, i.e. every stage of every step
+ * kept on a device tape of 64*M*(4*nsteps+1) bytes); each segment is replayed
+ * once with its stages taped, then swept backwards.  Requires the
+ * ForceConjugacy quirk off.  Leaves the handle's state undefined. */
+int ns2d_vjp(ns2d_t* h, const double* w0, double dt, long nsteps, const double* lam_T, double* grad_w0,
+             long ckpt_every);
+int ns2d_vjp_device(ns2d_t* h, const void* d_w0, double dt, long nsteps, const void* d_lam_T, void* d_grad_w0,
+                    long ckpt_every);
+
 const char* ns2d_last_error(void);
 
 #ifdef __cplusplus

@@ -1,0 +1,101 @@
+"""GPU: the hand-written discrete adjoint of the RK4CN map against torch
+autograd through a pure-torch FP64 restatement (CPU) and against central
+finite differences of the GPU forward map."""
+import numpy as np
+import pytest
+
+import ns2d_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def make(p):
+    import ns2d
+    cfg = ns2d.Config(Nx=p.Nx, Ny=p.Ny, nu=p.nu, hyper_flag=p.hyper_flag, hyper_pow=p.hyper_pow,
+                      alpha=p.alpha, drag_flag=p.drag_flag, drag_pow=p.drag_pow, filter=p.filter,
+                      quirks=p.quirks, check_every=0)
+    return ns2d.Solver(cfg)
+
+
+def torch_grad(p, w0, dt, n):
+    import torch
+    from ns2d_torch_ref import TorchRK4CN
+    T = TorchRK4CN(p)
+    w = torch.tensor(w0, requires_grad=True)
+    x = w
+    for _ in range(n):
+        x = T.step(x, dt)
+    J = T.enstrophy(x)
+    J.backward()
+    return float(J), w.grad.numpy(), x.detach().numpy()
+
+
+def enstrophy_grad(p, w):
+    kx, ky = O.wavenumbers(p.Nx, p.Ny)
+    c = np.full((1, p.Ny // 2 + 1), 2.0)
+    c[0, 0] = c[0, -1] = 1.0
+    g = 2.0 * c * w * (4 * np.pi ** 2) * (0.5 / float(p.Nx * p.Ny) ** 2)
+    g[0, 0] = 0.0
+    return g
+
+
+@pytest.mark.parametrize("Nx,Ny,n,ckpt", [(32, 32, 5, 0), (64, 32, 6, 4), (64, 64, 7, 3), (128, 128, 4, 0)])
+def test_vjp_matches_torch_autograd(Nx, Ny, n, ckpt):
+    print(f"vjp {Nx}x{Ny} n={n} ckpt={ckpt}", flush=True)
+    p = O.Params(Nx, Ny, nu=2e-3, hyper_flag=0, hyper_pow=1.0, filter="23", quirks=0)
+    w0 = O.ic_random_spectrum(p, seed=Nx + n, k0=4.0)
+    w0[Nx - 2, 0] += 0.2 * np.abs(w0).max()   # non-Hermitian ky=0 entry
+    dt = 2e-3
+    J, g_ref, wT = torch_grad(p, w0, dt, n)
+    with make(p) as s:
+        g = s.vjp(w0, dt, n, enstrophy_grad(p, wT), ckpt_every=ckpt)
+    assert np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref) < 1e-11
+
+
+def test_vjp_finite_differences():
+    """dZ(T)/dw0 along 8 seeded directions vs central differences of the GPU map."""
+    p = O.Params(128, 128, nu=1e-3, hyper_flag=0, hyper_pow=1.0, quirks=0)
+    w0 = O.ic_random_spectrum(p, seed=5, k0=6.0)
+    dt, n = 2e-3, 20
+    with make(p) as s:
+        def Z(w):
+            s.set_state(w)
+            s.spectral_step(dt, n)
+            return O.measures(s.get_state(), p)["enstrophy"], s.get_state()
+        _, wT = Z(w0)
+        g = s.vjp(w0, dt, n, enstrophy_grad(p, wT), ckpt_every=7)
+        rng = np.random.default_rng(11)
+        for _ in range(8):
+            d = rng.standard_normal(w0.shape) + 1j * rng.standard_normal(w0.shape)
+            d *= np.abs(w0) > 0
+            eps = 1e-6 * np.linalg.norm(w0) / np.linalg.norm(d)
+            fd = (Z(w0 + eps * d)[0] - Z(w0 - eps * d)[0]) / (2 * eps)
+            an = float(np.sum(g.real * d.real + g.imag * d.imag))
+            assert abs(fd - an) <= 1e-6 * abs(an) + 1e-14, (fd, an)
+
+
+def test_autograd_function_enstrophy():
+    import torch
+
+    import ns2d
+    from ns2d import autograd as nag
+    p = O.Params(64, 64, nu=2e-3, hyper_flag=0, hyper_pow=1.0, quirks=0)
+    w0 = O.ic_random_spectrum(p, seed=8, k0=4.0)
+    dt, n = 2e-3, 6
+    J_ref, g_ref, _ = torch_grad(p, w0, dt, n)
+    with make(p) as s:
+        w = torch.tensor(w0, device="cuda", requires_grad=True)
+        wT = nag.rk4cn(w, s, dt, n, 0)
+        J = nag.enstrophy(wT, p.Nx, p.Ny)
+        J.backward()
+        g = w.grad.detach().cpu().numpy()
+    assert abs(float(J) - J_ref) < 1e-12 * abs(J_ref)
+    assert np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref) < 1e-11
+
+
+def test_vjp_rejects_reference_conjugacy_quirk():
+    import ns2d
+    p = O.Params(32, 32, nu=1e-3)
+    with make(p) as s:
+        with pytest.raises(ns2d.NS2DError):
+            s.vjp(np.zeros((32, 17), complex), 1e-3, 2, np.zeros((32, 17), complex))
