@@ -1,4 +1,6 @@
-"""Builds libns2d_gpu.so (HIP kernels + C ABI) in-tree for gfx950 with hipcc."""
+"""Builds, in-tree: libns2d_gpu.so (HIP kernels + C ABI) for gfx950 with hipcc;
+libns2d_host.so (command line, initial conditions, HDF5 output; plain C) and
+the ns2d_solver executable (the reference-compatible driver) with gcc."""
 import os
 import subprocess
 import sys
@@ -10,6 +12,14 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = ["ns2d_kernels.hip", "ns2d_capi.cpp"]
+DRIVER = os.path.join(HERE, "driver")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+HOST_LIB = os.path.join(HERE, "libns2d_host.so")
+SOLVER = os.path.join(HERE, "ns2d_solver")
+CC = os.environ.get("CC", "gcc")
+# serial HDF5 of the image (1.10.6); the reference links -lhdf5 -lhdf5_hl
+HDF5_PREFIX = os.environ.get("NS2D_HDF5_PREFIX", "/opt/conda")
+CFLAGS = ["-O3", "-std=gnu99", "-W", "-Wall", "-fPIC"]
 
 
 def build(verbose: bool = False) -> str:
@@ -28,8 +38,29 @@ def build(verbose: bool = False) -> str:
     subprocess.run(cmd, check=True)
     for o in objs:
         os.remove(o)
+    build_driver(verbose)
     return LIB
 
 
+def build_driver(verbose: bool = False) -> str:
+    inc = ["-I" + INCLUDE, "-I" + DRIVER, "-I" + os.path.join(HDF5_PREFIX, "include")]
+    h5 = ["-L" + os.path.join(HDF5_PREFIX, "lib"), "-lhdf5_hl", "-lhdf5",
+          "-Wl,-rpath," + os.path.join(HDF5_PREFIX, "lib")]
+    srcs = [os.path.join(DRIVER, f) for f in ("ns2d_cli.c", "ns2d_ics.c", "ns2d_h5.c")]
+    # HDF5's directory is a run path of libns2d_host.so only: that directory
+    # also holds an older libstdc++ that the HIP runtime must not pick up
+    cmds = [[CC] + CFLAGS + inc + ["-shared", "-o", HOST_LIB] + srcs + h5 + ["-lm"],
+            [CC] + CFLAGS + inc + [os.path.join(DRIVER, "ns2d_main.c"), "-o", SOLVER, "-L" + HERE,
+                                   "-lns2d_host", "-lns2d_gpu", "-Wl,-rpath,$ORIGIN", "-lm"]]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    return SOLVER
+
+
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    if "--driver" in sys.argv:
+        print(build_driver(verbose="-v" in sys.argv))
+    else:
+        print(build(verbose="-v" in sys.argv))

@@ -829,6 +829,34 @@ int ns2d_r2c(ns2d_t* h, const double* w, double* w_hat) {
   return NS2D_OK;
 }
 
+int ns2d_c2r(ns2d_t* h, const double* w_hat, double* w) {
+  if (!h || !w_hat || !w) return fail(NS2D_ERR_INVALID, "null argument");
+  HIPCHK(hipSetDevice(h->p.device));
+  const size_t n = (size_t)h->p.nx * h->p.ny;
+  if (!h->wreal) HIPCHK(hipMalloc((void**)&h->wreal, sizeof(double) * n));
+  HIPCHK(hipMemcpy2DAsync(h->sbuf, h->ld * sizeof(double2), w_hat, h->nyf * sizeof(double2), h->nyf * sizeof(double2),
+                          h->p.nx, hipMemcpyHostToDevice, h->stream));
+  KPassArgs a = kbase(h);
+  a.mode = KP_INV;
+  a.epi = EPI_FROM_INPUT;
+  a.sin = h->sbuf;
+  a.conj_dst = nullptr;  // the state's ky=0 hand-off is left alone
+  int rc = run_kpass(h, a);
+  if (rc) return rc;
+  h->inv_valid = false;  // G1..G4 now hold the transforms of w_hat
+  RPassArgs r = rbase(h, RP_W);
+  r.wreal = h->wreal;
+  r.wscale = 1.0;
+  rc = run_rpass(h, r);
+  if (rc) return rc;
+  // the row pass also folded max|w| into the step check's accumulator
+  const unsigned long long zero = 0ull;
+  HIPCHK(hipMemcpyAsync(h->stat + STAT_WMAX_CUR, &zero, sizeof zero, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(w, h->wreal, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return NS2D_OK;
+}
+
 int ns2d_get_info(ns2d_t* h, ns2d_info_t* info) {
   if (!h || !info) return fail(NS2D_ERR_INVALID, "null argument");
   HIPCHK(hipSetDevice(h->p.device));

@@ -62,7 +62,7 @@ class _Info(ctypes.Structure):
 
 EXPORTS = ["ns2d_params_default", "ns2d_create", "ns2d_destroy", "ns2d_set_state", "ns2d_get_state",
            "ns2d_set_state_device", "ns2d_get_state_device", "ns2d_step", "ns2d_rhs", "ns2d_measures",
-           "ns2d_max_vort", "ns2d_real_vort", "ns2d_r2c", "ns2d_get_info", "ns2d_sync",
+           "ns2d_max_vort", "ns2d_real_vort", "ns2d_r2c", "ns2d_c2r", "ns2d_get_info", "ns2d_sync",
            "ns2d_get_stream", "ns2d_profile_steps", "ns2d_variant_count", "ns2d_variant_info",
            "ns2d_set_variant", "ns2d_ens_create", "ns2d_ens_member", "ns2d_ens_step", "ns2d_ens_sync",
            "ns2d_ens_destroy", "ns2d_vjp", "ns2d_vjp_device", "ns2d_last_error"]
@@ -94,6 +94,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ns2d_max_vort": ([vp, dp], i),
         "ns2d_real_vort": ([vp, dp], i),
         "ns2d_r2c": ([vp, dp, dp], i),
+"ns2d_c2r": ([vp, dp, dp], i),
         "ns2d_get_info": ([vp, P(_Info)], i),
         "ns2d_sync": ([vp], i),
         "ns2d_get_stream": ([vp, P(vp)], i),
@@ -280,6 +281,14 @@ class Solver:
         assert x.shape == (self.Nx, self.Ny)
         out = np.empty((self.Nx, self.Nyf), dtype=np.complex128)
         _check(_lib.ns2d_r2c(self._h, _dptr(x), _dptr(out.view(np.float64))))
+        return out
+
+    def c2r(self, w_hat: np.ndarray) -> np.ndarray:
+        """Unnormalised inverse transform (FFTW c2r conventions); the state is untouched."""
+        x = np.ascontiguousarray(w_hat, dtype=np.complex128)
+        assert x.shape == (self.Nx, self.Nyf)
+        out = np.empty((self.Nx, self.Ny), dtype=np.float64)
+        _check(_lib.ns2d_c2r(self._h, _dptr(x.view(np.float64)), _dptr(out)))
         return out
 
     def info(self) -> dict:

@@ -138,6 +138,12 @@ int ns2d_real_vort(ns2d_t* h, double* w);
  * (fftw_mpi_execute_dft_r2c, solver.c:1178): w [Nx][Ny] -> w_hat [Nx][Ny/2+1]. */
 int ns2d_r2c(ns2d_t* h, const double* w, double* w_hat);
 
+/* Unnormalised inverse transform of a host spectrum, as
+ * fftw_mpi_execute_dft_c2r (solver.c:1407 DECAY_TURB_ALT, :2409 GetTimestep):
+ * w_hat [Nx][Ny/2+1] -> w [Nx][Ny]; the imaginary parts of the ky = 0 and
+ * ky = Ny/2 entries enter as FFTW's c2r lets them.  Leaves the state alone. */
+int ns2d_c2r(ns2d_t* h, const double* w_hat, double* w);
+
 int ns2d_get_info(ns2d_t* h, ns2d_info_t* info);
 int ns2d_sync(ns2d_t* h);
 /* The handle's hipStream_t, for callers that order their own work after it. */
