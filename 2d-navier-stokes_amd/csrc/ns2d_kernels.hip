@@ -248,7 +248,8 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
     unsigned long long* st = a.stat;
     const unsigned long long m = st[STAT_WMAX_CUR];
     st[STAT_WMAX_LAST] = m;
-    if (__longlong_as_double((long long)m) >= MAX_VORT_LIM && st[STAT_FIRST_BAD] == ~0ull)
+    // a NaN maximum also counts (the reference's >= test would let it pass)
+    if (!(__longlong_as_double((long long)m) < MAX_VORT_LIM) && st[STAT_FIRST_BAD] == ~0ull)
       st[STAT_FIRST_BAD] = st[STAT_ITER];
     st[STAT_WMAX_CUR] = 0ull;
   }

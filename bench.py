@@ -8,8 +8,9 @@ normalised to E = 0.5, nu = 5e-5 (nu k^2 viscosity), Hou-Li filter in the
 reference's integer-division form, RK4CN, dt = 5e-4, with the reference's
 per-step max|w| blow-up check.  A "step" is one full RK4CN time step (four
 nonlinear right-hand sides).  The state is resident in HBM before timing
-starts.  The same step is also timed on a 4096^2 grid (reported under
-"extra").
+starts.  Reported under "extra": the same step on a 4096^2 grid, the
+64-member uneven-size ensemble against sequential plain solves, and the
+discrete adjoint's cost relative to a forward solve (512^2, 2000 steps).
 
 The roofline object covers the dominant kernel, timed live with HIP events
 recorded on the solver's own stream around every launch; its algorithmic
@@ -77,6 +78,101 @@ def time_gpu(nx, ny, steps, warmup):
     return (t1 - t0), kms, rms, m
 
 
+def cfl_dt(w_hat, nx, ny, nu, cfl=0.5):
+    """GetTimestep's CFL formula (solver.c:2381-2450) on the host with every
+    mode's velocity: cfl / (pi (max|u|/dx + max|v|/dy) + pi^2 nu (1/dx^2 + 1/dy^2))."""
+    from ns2d import ics
+    kx, ky = ics.wavenumbers(nx, ny)
+    k2 = (kx * kx + ky * ky).astype(np.float64)
+    inv = np.where(k2 > 0, 1.0 / np.where(k2 > 0, k2, 1.0), 0.0)
+    u = np.fft.irfft2(1j * ky * inv * w_hat, s=(nx, ny), norm="forward") / (nx * ny)
+    v = np.fft.irfft2(-1j * kx * inv * w_hat, s=(nx, ny), norm="forward") / (nx * ny)
+    dx, dy = 2 * math.pi / nx, 2 * math.pi / ny
+    conv = np.max(math.pi * (np.abs(u) / dx + np.abs(v) / dy))
+    return cfl / (conv + math.pi ** 2 * nu * (1 / dx ** 2 + 1 / dy ** 2))
+
+
+def ensemble_leg(steps=500, seed=2024):
+    """64 decaying-turbulence members of uneven size (24 x 256^2, 24 x 512^2,
+    16 x 1024^2), k0 ~ U[4, 12] per member, 2/3 filter, RK4CN, dt per member
+    from CFL 0.5 at t = 0; one ensemble launch sequence vs the same members
+    solved one after another with the plain solver."""
+    import ns2d
+    from ns2d import ics
+    rng = np.random.default_rng(seed)
+    sizes = [256] * 24 + [512] * 24 + [1024] * 16
+    nu = 5e-5
+    cfgs, ws, dts = [], [], []
+    for i, n in enumerate(sizes):
+        k0 = float(rng.uniform(4.0, 12.0))
+        w0 = ics.random_phase(n, n, seed=seed + i, k0=k0, e0=0.5, filt="23")
+        cfgs.append(ns2d.Config(Nx=n, Ny=n, nu=nu, hyper_flag=0, hyper_pow=1.0, alpha=0.0, drag_flag=0,
+                                drag_pow=0.0, filter="23", integrator="RK4CN", check_every=1))
+        ws.append(w0)
+        dts.append(cfl_dt(w0, n, n, nu))
+    member_steps = steps * len(sizes)
+    with ns2d.Ensemble(cfgs) as e:
+        for m, w in zip(e.members, ws):
+            m.set_state(w)
+        e.step(dts, 2)
+        e.sync()
+        for m, w in zip(e.members, ws):
+            m.set_state(w)
+        t0 = time.perf_counter()
+        e.step(dts, steps)
+        e.sync()
+        t_ens = time.perf_counter() - t0
+    t_seq = 0.0
+    for c, w, dt in zip(cfgs, ws, dts):
+        with ns2d.Solver(c) as s:
+            s.set_state(w)
+            s.spectral_step(dt, 2)
+            s.sync()
+            s.set_state(w)
+            t0 = time.perf_counter()
+            s.spectral_step(dt, steps)
+            s.sync()
+            t_seq += time.perf_counter() - t0
+    return {"ensemble_members": len(sizes), "ensemble_steps_each": steps,
+            "ensemble_member_steps_per_s": member_steps / t_ens,
+            "sequential_member_steps_per_s": member_steps / t_seq,
+            "ensemble_vs_sequential": t_seq / t_ens}
+
+
+def adjoint_leg(n=512, steps=2000, ckpt=100, seed=5):
+    """dZ(T)/dw0 of 2000 RK4CN steps at 512^2 (nu = 1e-4, dt = 1e-3, 2/3,
+    E(k) ~ k^4 exp(-(k/6)^2)), checkpoint every 100 steps with recompute;
+    cost relative to one forward solve of the same length."""
+    import ns2d
+    from ns2d import ics
+    nu, dt = 1e-4, 1e-3
+    cfg = ns2d.Config(Nx=n, Ny=n, nu=nu, hyper_flag=0, hyper_pow=1.0, alpha=0.0, drag_flag=0, drag_pow=0.0,
+                      filter="23", integrator="RK4CN", quirks=0, check_every=0)
+    w0 = ics.random_phase(n, n, seed=seed, k0=6.0, e0=0.5, filt="23")
+    with ns2d.Solver(cfg) as s:
+        s.set_state(w0)
+        s.spectral_step(dt, 10)
+        s.sync()
+        s.set_state(w0)
+        t0 = time.perf_counter()
+        s.spectral_step(dt, steps)
+        s.sync()
+        t_fwd = time.perf_counter() - t0
+        wT = s.get_state()
+        kx, ky = ics.wavenumbers(n, n)
+        c = np.full((1, n // 2 + 1), 2.0)
+        c[0, 0] = c[0, -1] = 1.0
+        lam = 2.0 * c * wT * (4 * np.pi ** 2) * (0.5 / float(n * n) ** 2)
+        lam[0, 0] = 0.0
+        s.vjp(w0, dt, 20, lam, ckpt_every=10)   # warm-up (allocations)
+        t0 = time.perf_counter()
+        g = s.vjp(w0, dt, steps, lam, ckpt_every=ckpt)
+        t_adj = time.perf_counter() - t0
+    return {"adjoint_grid": n, "adjoint_steps": steps, "adjoint_ckpt_every": ckpt,
+            "forward_s": t_fwd, "vjp_s": t_adj, "adjoint_over_forward": t_adj / t_fwd,
+            "grad_norm": float(np.linalg.norm(g))}
+
+
 def cpu_baseline(nx, ny, budget_s=20.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ns2d_oracle as O
@@ -108,6 +204,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-4096", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU (slab-decomposed) runs are not "
@@ -136,6 +234,10 @@ def main():
                       "rk4_4096_kpass_avg_ms": k4, "rk4_4096_rpass_avg_ms": r4,
                       "rk4_4096_step_alg_GBps": ab4["step"] * s4 / 1e9,
                       "rk4_4096_step_model928_frac": ab4["step_model"] * s4 / 1e9 / HBM_PEAK_GBS})
+    if not args.no_ensemble:
+        extra.update(ensemble_leg())
+    if not args.no_adjoint:
+        extra.update(adjoint_leg())
     out = {
         "metric": "RK4 steps/s at 1024^2 FP64 (RK4CN, decaying turbulence)",
         "value": sps,
