@@ -70,6 +70,21 @@ EXPORTS = ["ns2d_params_default", "ns2d_create", "ns2d_destroy", "ns2d_set_state
 _lib = None
 
 
+def _one_hip_runtime() -> None:
+    """A process gets one HIP/HSA runtime.  torch's wheel carries its own
+    runtime (libamdhip64.so with the same soname as /opt/rocm's, and its own
+    libhsa-runtime64.so); when libns2d_gpu.so loads /opt/rocm's first and torch
+    is imported later, torch brings a second HSA runtime and one of the two
+    sees no GPU.  Importing torch first makes this library bind to torch's
+    runtime, so both work in one process.  NS2D_NO_TORCH=1 skips it."""
+    if os.environ.get("NS2D_NO_TORCH") == "1":
+        return
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
 def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     """Load libns2d_gpu.so (raises OSError when it has not been built)."""
     global _lib
@@ -77,6 +92,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         return _lib
     if not os.path.exists(path):
         raise OSError(f"{path} not built: run __graft_entry__.build()")
+    _one_hip_runtime()
     lib = ctypes.CDLL(path)
     P = ctypes.POINTER
     vp, dp, i, d, l = ctypes.c_void_p, P(ctypes.c_double), ctypes.c_int, ctypes.c_double, ctypes.c_long
@@ -94,7 +110,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ns2d_max_vort": ([vp, dp], i),
         "ns2d_real_vort": ([vp, dp], i),
         "ns2d_r2c": ([vp, dp, dp], i),
-"ns2d_c2r": ([vp, dp, dp], i),
+        "ns2d_c2r": ([vp, dp, dp], i),
         "ns2d_get_info": ([vp, P(_Info)], i),
         "ns2d_sync": ([vp], i),
         "ns2d_get_stream": ([vp, P(vp)], i),

@@ -53,7 +53,9 @@ def test_vjp_matches_torch_autograd(Nx, Ny, n, ckpt):
 
 
 def test_vjp_finite_differences():
-    """dZ(T)/dw0 along 8 seeded directions vs central differences of the GPU map."""
+    """dZ(T)/dw0 along 16 seeded directions against central differences of
+    the GPU map (Richardson-extrapolated in the step, so the O(eps^2) term
+    drops out): agreement to 1e-6 relative."""
     p = O.Params(128, 128, nu=1e-3, hyper_flag=0, hyper_pow=1.0, quirks=0)
     w0 = O.ic_random_spectrum(p, seed=5, k0=6.0)
     dt, n = 2e-3, 20
@@ -61,17 +63,22 @@ def test_vjp_finite_differences():
         def Z(w):
             s.set_state(w)
             s.spectral_step(dt, n)
-            return O.measures(s.get_state(), p)["enstrophy"], s.get_state()
+            wT = s.get_state()
+            return O.measures(wT, p)["enstrophy"], wT
         _, wT = Z(w0)
         g = s.vjp(w0, dt, n, enstrophy_grad(p, wT), ckpt_every=7)
         rng = np.random.default_rng(11)
-        for _ in range(8):
-            d = rng.standard_normal(w0.shape) + 1j * rng.standard_normal(w0.shape)
-            d *= np.abs(w0) > 0
-            eps = 1e-6 * np.linalg.norm(w0) / np.linalg.norm(d)
-            fd = (Z(w0 + eps * d)[0] - Z(w0 - eps * d)[0]) / (2 * eps)
+        amp = np.abs(w0)
+        for k in range(16):
+            # directions with the initial condition's spectral support and
+            # amplitudes, random phases (non-Hermitian on the ky=0 column too)
+            d = amp * np.exp(2j * np.pi * rng.random(w0.shape))
+            h = 1e-3 * np.linalg.norm(w0) / np.linalg.norm(d)
+            D = lambda e: (Z(w0 + e * d)[0] - Z(w0 - e * d)[0]) / (2 * e)
+            fd = (4.0 * D(h / 2) - D(h)) / 3.0
             an = float(np.sum(g.real * d.real + g.imag * d.imag))
-            assert abs(fd - an) <= 1e-6 * abs(an) + 1e-14, (fd, an)
+            print(f"direction {k}: fd {fd:.12e} adjoint {an:.12e}", flush=True)
+            assert abs(fd - an) <= 1e-6 * abs(an), (k, fd, an)
 
 
 @pytest.mark.parametrize("device", ["cpu", "cuda"])
