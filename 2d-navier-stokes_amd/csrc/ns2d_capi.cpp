@@ -363,6 +363,12 @@ int ns2d_variant_info(int pass, int n, int idx, int* shape4) {
   return NS2D_OK;
 }
 
+int ns2d_variant_min_waves(int pass, int n, int idx) {
+  if (pass == NS2D_PASS_COLUMN) return (idx >= 0 && idx < kpass_variant_count(n)) ? 1 : -1;
+  if (pass == NS2D_PASS_ROW) return rpass_variant_min_waves(n, idx);
+  return -1;
+}
+
 int ns2d_set_variant(ns2d_t* h, int pass, int idx) {
   if (!h) return fail(NS2D_ERR_INVALID, "null handle");
   if (pass == NS2D_PASS_COLUMN) {

@@ -441,13 +441,14 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   }
 }
 
-template <int NY, int TLR, int PAIRS>
-__global__ __launch_bounds__(TLR * PAIRS) void rpass_kernel(RPassArgs a) {
+// W: minimum waves per SIMD the register allocation must allow (1 = no limit)
+template <int NY, int TLR, int PAIRS, int W>
+__global__ __launch_bounds__(TLR * PAIRS, W) void rpass_kernel(RPassArgs a) {
   rpass_body<NY, TLR, PAIRS>(a, (int)blockIdx.x);
 }
 
-template <int NY, int TLR, int PAIRS>
-__global__ __launch_bounds__(TLR * PAIRS) void rpass_batched(const RPassArgs* __restrict__ tab, int per_member) {
+template <int NY, int TLR, int PAIRS, int W>
+__global__ __launch_bounds__(TLR * PAIRS, W) void rpass_batched(const RPassArgs* __restrict__ tab, int per_member) {
   const int m = (int)blockIdx.x / per_member;
   rpass_body<NY, TLR, PAIRS>(tab[m], (int)blockIdx.x - m * per_member);
 }
@@ -925,39 +926,42 @@ hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s) 
 // per thread), PAIRS row pairs per workgroup; the first listed for each NY is
 // the default.
 #define NS2D_RPASS_VARIANTS(X) \
-  X(8, 1, 64)                  \
-  X(16, 1, 64)                 \
-  X(32, 2, 32)                 \
-  X(32, 4, 16)                 \
-  X(64, 4, 16)                 \
-  X(64, 8, 8)                  \
-  X(128, 8, 8)                 \
-  X(128, 16, 4)                \
-  X(128, 32, 2)                \
-  X(256, 16, 4)                \
-  X(256, 32, 2)                \
-  X(256, 64, 1)                \
-  X(512, 32, 2)                \
-  X(512, 64, 1)                \
-  X(512, 128, 1)               \
-  X(1024, 64, 1)               \
-  X(1024, 128, 1)              \
-  X(1024, 256, 1)              \
-  X(2048, 128, 1)              \
-  X(2048, 256, 1)              \
-  X(2048, 512, 1)              \
-  X(4096, 256, 1)              \
-  X(4096, 512, 1)              \
-  X(4096, 1024, 1)
+  X(8, 1, 64, 1)                  \
+  X(16, 1, 64, 1)                 \
+  X(32, 2, 32, 1)                 \
+  X(32, 4, 16, 1)                 \
+  X(64, 4, 16, 1)                 \
+  X(64, 8, 8, 1)                  \
+  X(128, 8, 8, 1)                 \
+  X(128, 16, 4, 1)                \
+  X(128, 32, 2, 1)                \
+  X(256, 16, 4, 1)                \
+  X(256, 32, 2, 1)                \
+  X(256, 64, 1, 1)                \
+  X(512, 32, 2, 1)                \
+  X(512, 64, 1, 1)                \
+  X(512, 128, 1, 1)               \
+  X(1024, 64, 1, 1)               \
+  X(1024, 128, 1, 1)              \
+  X(1024, 256, 1, 1)              \
+  X(2048, 128, 1, 1)            \
+  X(2048, 128, 1, 2)              \
+  X(2048, 256, 1, 1)              \
+  X(2048, 512, 1, 1)              \
+  X(4096, 256, 1, 1)            \
+  X(4096, 256, 1, 2)            \
+  X(4096, 512, 1, 4)              \
+  X(4096, 512, 1, 1)              \
+  X(4096, 1024, 1, 1)
 
 struct RVariant {
-  int NY, TLR, PAIRS;
+  int NY, TLR, PAIRS, W;
   void (*kern)(RPassArgs);
   void (*batched)(const RPassArgs*, int);
 };
 
-#define NS2D_RV_ENTRY(NY_, TLR_, PAIRS_) \
-  {NY_, TLR_, PAIRS_, rpass_kernel<NY_, TLR_, PAIRS_>, rpass_batched<NY_, TLR_, PAIRS_>},
+#define NS2D_RV_ENTRY(NY_, TLR_, PAIRS_, W_) \
+  {NY_, TLR_, PAIRS_, W_, rpass_kernel<NY_, TLR_, PAIRS_, W_>, rpass_batched<NY_, TLR_, PAIRS_, W_>},
 static const RVariant rvariants[] = {NS2D_RPASS_VARIANTS(NS2D_RV_ENTRY)};
 #undef NS2D_RV_ENTRY
 
@@ -992,6 +996,11 @@ int rpass_variant_info(int ny, int idx, int* TLR, int* E, int* T, int* PAIRS) {
   *T = v->TLR * v->PAIRS;
   *PAIRS = v->PAIRS;
   return 0;
+}
+
+int rpass_variant_min_waves(int ny, int idx) {
+  const RVariant* v = find_rvariant(ny, idx);
+  return v ? v->W : -1;
 }
 
 hipError_t launch_rpass(int nx, int ny, int variant, const RPassArgs& a, hipStream_t s) {

@@ -167,6 +167,9 @@ int ns2d_profile_steps(ns2d_t* h, double dt, long nsteps, double* kpass_ms, doub
 #define NS2D_PASS_ROW 1
 int ns2d_variant_count(int pass, int n);
 int ns2d_variant_info(int pass, int n, int idx, int* shape4);
+/* The register budget a shape was compiled for, as the minimum number of
+ * waves per SIMD it must allow (1: no constraint). */
+int ns2d_variant_min_waves(int pass, int n, int idx);
 int ns2d_set_variant(ns2d_t* h, int pass, int idx);
 
 /* Ensemble (no counterpart in the reference, which runs one realisation per
