@@ -1279,7 +1279,9 @@ int vjp_impl(ns2d_handle* h, double dt, long n, long ckpt_every, double2** res) 
   h->hyper_eff = h->p.hyper_flag;
   if (!lam0 || !lam1) return fail(NS2D_ERR_HIP, "coefficient table allocation failed");
   auto lam_of = [&](long step) { return step == 0 ? lam0 : lam1; };
-  // forward sweep: checkpoints, the last segment taped
+  // forward sweep: checkpoints, the last segment taped.  w_0 was just placed
+  // in the state buffer, so G1..G4 of an earlier state must not be reused.
+  h->inv_valid = false;
   h->iters = 0;
   for (long seg = 0; seg < S; ++seg) {
     HIPCHK(hipMemcpyAsync(h->ckpt + (size_t)seg * h->M, h->w[h->wcur], fb, hipMemcpyDeviceToDevice, h->stream));

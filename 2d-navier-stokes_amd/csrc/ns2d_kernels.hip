@@ -832,6 +832,7 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(512, 8, 256, 4)            \
   X(512, 8, 512, 2)            \
   X(512, 4, 256, 2)            \
+  X(1024, 2, 256, 2)           \
   X(1024, 4, 256, 2)           \
   X(1024, 4, 256, 1)           \
   X(1024, 4, 256, 4)           \
@@ -839,7 +840,6 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(1024, 2, 128, 4)           \
   X(1024, 1, 64, 4)            \
   X(1024, 4, 512, 2)           \
-  X(1024, 2, 256, 2)           \
   X(1024, 4, 512, 1)           \
   X(2048, 2, 256, 2)           \
   X(2048, 2, 256, 1)           \
@@ -926,32 +926,32 @@ hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s) 
 // per thread), PAIRS row pairs per workgroup; the first listed for each NY is
 // the default.
 #define NS2D_RPASS_VARIANTS(X) \
-  X(8, 1, 64, 1)                  \
-  X(16, 1, 64, 1)                 \
-  X(32, 2, 32, 1)                 \
-  X(32, 4, 16, 1)                 \
-  X(64, 4, 16, 1)                 \
-  X(64, 8, 8, 1)                  \
-  X(128, 8, 8, 1)                 \
-  X(128, 16, 4, 1)                \
-  X(128, 32, 2, 1)                \
-  X(256, 16, 4, 1)                \
-  X(256, 32, 2, 1)                \
-  X(256, 64, 1, 1)                \
-  X(512, 32, 2, 1)                \
-  X(512, 64, 1, 1)                \
-  X(512, 128, 1, 1)               \
-  X(1024, 64, 1, 1)               \
-  X(1024, 128, 1, 1)              \
-  X(1024, 256, 1, 1)              \
-  X(2048, 128, 1, 1)            \
-  X(2048, 128, 1, 2)              \
-  X(2048, 256, 1, 1)              \
-  X(2048, 512, 1, 1)              \
-  X(4096, 256, 1, 1)            \
-  X(4096, 256, 1, 2)            \
-  X(4096, 512, 1, 4)              \
-  X(4096, 512, 1, 1)              \
+  X(8, 1, 64, 1)               \
+  X(16, 1, 64, 1)              \
+  X(32, 2, 32, 1)              \
+  X(32, 4, 16, 1)              \
+  X(64, 4, 16, 1)              \
+  X(64, 8, 8, 1)               \
+  X(128, 8, 8, 1)              \
+  X(128, 16, 4, 1)             \
+  X(128, 32, 2, 1)             \
+  X(256, 16, 4, 1)             \
+  X(256, 32, 2, 1)             \
+  X(256, 64, 1, 1)             \
+  X(512, 32, 2, 1)             \
+  X(512, 64, 1, 1)             \
+  X(512, 128, 1, 1)            \
+  X(1024, 256, 1, 1)           \
+  X(1024, 64, 1, 1)            \
+  X(1024, 128, 1, 1)           \
+  X(2048, 128, 1, 1)           \
+  X(2048, 128, 1, 2)           \
+  X(2048, 256, 1, 1)           \
+  X(2048, 512, 1, 1)           \
+  X(4096, 512, 1, 4)           \
+  X(4096, 256, 1, 1)           \
+  X(4096, 256, 1, 2)           \
+  X(4096, 512, 1, 1)           \
   X(4096, 1024, 1, 1)
 
 struct RVariant {

@@ -204,6 +204,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-4096", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="end with a 1 GiB device-to-device copy (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
@@ -261,6 +263,13 @@ def main():
         "extra": extra,
     }
     print(json.dumps(out), flush=True)
+    if args.calib_copy:
+        import torch
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print("calibration copy: 1 GiB device to device", flush=True)
 
 
 if __name__ == "__main__":

@@ -102,6 +102,20 @@ def test_autograd_function_enstrophy(device):
     assert np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref) < 1e-11
 
 
+def test_vjp_after_forward_steps_on_the_same_handle():
+    """A handle that has stepped another state before keeps no stale
+    transforms into the adjoint's forward sweep."""
+    p = O.Params(64, 64, nu=1e-3, hyper_flag=0, hyper_pow=1.0, quirks=0)
+    w0 = O.ic_random_spectrum(p, seed=9, k0=6.0)
+    dt, n = 2e-3, 9
+    J, g_ref, wT = torch_grad(p, w0, dt, n)
+    with make(p) as s:
+        s.set_state(O.ic_random_spectrum(p, seed=10, k0=3.0))
+        s.spectral_step(dt, 5)
+        g = s.vjp(w0, dt, n, enstrophy_grad(p, wT), ckpt_every=4)
+    assert np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref) < 1e-11
+
+
 def test_vjp_rejects_reference_conjugacy_quirk():
     import ns2d
     p = O.Params(32, 32, nu=1e-3)

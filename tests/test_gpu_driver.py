@@ -37,7 +37,8 @@ def test_c2r_r2c_entry_points():
     Nx, Ny = 64, 32
     rng = np.random.default_rng(2)
     f = rng.standard_normal((Nx, Ny // 2 + 1)) + 1j * rng.standard_normal((Nx, Ny // 2 + 1))
-    with ns2d.Solver(ns2d.Config(Nx=Nx, Ny=Ny)) as s:
+    cfg = ns2d.Config(Nx=Nx, Ny=Ny, nu=1e-3, hyper_flag=0, hyper_pow=1.0, alpha=0.0, drag_flag=0, drag_pow=0.0)
+    with ns2d.Solver(cfg) as s:
         w0 = O.ic_random_spectrum(O.Params(Nx, Ny), seed=1)
         s.set_state(w0)
         got = s.c2r(f)          # non-Hermitian input: FFTW's c2r projection
