@@ -222,6 +222,11 @@ def main():
     else:
         dom, dom_ms, dom_bytes = "rpass", rms, ab["rpass"]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(f"{dom}_{nx}")
     extra = {
         "kpass_avg_ms": kms, "rpass_avg_ms": rms,
         "step_alg_GBps": ab["step"] * sps / 1e9,
@@ -257,7 +262,7 @@ def main():
                                f"form), nu=5e-5, dt=5e-4, per-step max|w| check",
                    "grid": [nx, ny], "integrator": "RK4CN", "filter": "houli_ref", "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(nx, ny, args.cpu_budget),
         "extra": extra,
