@@ -85,6 +85,16 @@ struct ns2d_handle {
   double2* ckpt = nullptr;     // [segment][M]
   long ckpt_n = 0;
   double2 *lamp[2] = {nullptr, nullptr}, *lacc = nullptr, *lbuf = nullptr, *bbuf[4] = {nullptr, nullptr, nullptr, nullptr};
+  // hipGraphs of one full adjoint segment (recompute + backward sweep), one
+  // per parity of the state buffer and of the adjoint ping-pong at its start
+  struct AdjGraph {
+    hipGraphExec_t exec = nullptr;
+    double dt = 0.0;
+    long C = 0;
+    int wcur = -1, lc = -1;
+    const double* lam = nullptr;
+    const double2* tape = nullptr;
+  } adj_graph[2];
   // profiling
   bool profiling = false;
   std::vector<hipEvent_t> ev;
@@ -363,10 +373,14 @@ int ns2d_variant_info(int pass, int n, int idx, int* shape4) {
   return NS2D_OK;
 }
 
-int ns2d_variant_min_waves(int pass, int n, int idx) {
-  if (pass == NS2D_PASS_COLUMN) return (idx >= 0 && idx < kpass_variant_count(n)) ? 1 : -1;
-  if (pass == NS2D_PASS_ROW) return rpass_variant_min_waves(n, idx);
-  return -1;
+int ns2d_variant_extra(int pass, int n, int idx, int* extra4) {
+  if (!extra4) return fail(NS2D_ERR_INVALID, "null argument");
+  int rc = -1;
+  if (pass == NS2D_PASS_COLUMN) rc = kpass_variant_extra(n, idx, &extra4[0], &extra4[1], &extra4[2]);
+  if (pass == NS2D_PASS_ROW) rc = rpass_variant_extra(n, idx, &extra4[0], &extra4[1], &extra4[2]);
+  if (rc) return fail(NS2D_ERR_INVALID, "no variant %d of pass %d for n=%d", idx, pass, n);
+  extra4[3] = 0;
+  return NS2D_OK;
 }
 
 int ns2d_set_variant(ns2d_t* h, int pass, int idx) {
@@ -557,6 +571,8 @@ int ns2d_destroy(ns2d_t* h) {
   if (h->wreal) (void)hipFree(h->wreal);
   if (h->rbuf) (void)hipFree(h->rbuf);
   drop_graph(h);
+  for (auto& g : h->adj_graph)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return NS2D_OK;
@@ -1281,32 +1297,34 @@ int vjp_impl(ns2d_handle* h, double dt, long n, long ckpt_every, double2** res) 
   auto lam_of = [&](long step) { return step == 0 ? lam0 : lam1; };
   // forward sweep: checkpoints, the last segment taped.  w_0 was just placed
   // in the state buffer, so G1..G4 of an earlier state must not be reused.
+  // The untaped segments run as ordinary (graph-replayed) steps.
   h->inv_valid = false;
   h->iters = 0;
+  const int check0 = h->p.check_every;
+  h->p.check_every = 0;
   for (long seg = 0; seg < S; ++seg) {
     HIPCHK(hipMemcpyAsync(h->ckpt + (size_t)seg * h->M, h->w[h->wcur], fb, hipMemcpyDeviceToDevice, h->stream));
     const long cnt = (seg == S - 1) ? n - seg * C : C;
-    if (seg == S - 1) {
-      rc = taped_forward(h, dt, cnt);
-    } else {
-      rc = prepare_inv(h);
-      for (long j = 0; j < cnt && rc == NS2D_OK; ++j) rc = rk4cn_step(h, dt, false);
+    rc = (seg == S - 1) ? taped_forward(h, dt, cnt) : step_impl(h, dt, cnt);
+    if (rc) {
+      h->p.check_every = check0;
+      return rc;
     }
-    if (rc) return rc;
     h->inv_valid = false;
   }
+  h->p.check_every = check0;
   // backward sweep
   int lc = 0;   // lamp[lc] holds dJ/dw at the current step boundary
   bool started = false;
-  for (long seg = S - 1; seg >= 0; --seg) {
-    const long cnt = (seg == S - 1) ? n - seg * C : C;
+  // one segment: recompute its stages onto the tape (all but the last
+  // segment, taped above), then the adjoint stages backwards over its steps
+  auto segment = [&](long seg, long cnt) -> int {
     if (seg != S - 1) {
-      HIPCHK(hipMemcpyAsync(h->w[h->wcur], h->ckpt + (size_t)seg * h->M, fb, hipMemcpyDeviceToDevice, h->stream));
       h->iters = seg * C;
       h->hyper_eff = h->p.hyper_flag;
       if (h->iters > 0) after_rk_update(h);
-      rc = taped_forward(h, dt, cnt);
-      if (rc) return rc;
+      int r = taped_forward(h, dt, cnt);
+      if (r) return r;
     }
     for (long j = cnt - 1; j >= 0; --j) {
       const long ng = seg * C + j;
@@ -1355,6 +1373,54 @@ int vjp_impl(ns2d_handle* h, double dt, long n, long ckpt_every, double2** res) 
       }
       lc ^= 1;
     }
+    return NS2D_OK;
+  };
+  for (long seg = S - 1; seg >= 0; --seg) {
+    const long cnt = (seg == S - 1) ? n - seg * C : C;
+    if (seg != S - 1)
+      HIPCHK(hipMemcpyAsync(h->w[h->wcur], h->ckpt + (size_t)seg * h->M, fb, hipMemcpyDeviceToDevice, h->stream));
+    // full middle segments repeat the same launches: one capture per
+    // (state buffer, adjoint buffer) parity, replayed for the others
+    const bool graphable = h->use_graph && !h->profiling && started && seg != S - 1 && seg != 0 && C >= 2;
+    if (!graphable) {
+      rc = segment(seg, cnt);
+      if (rc) return rc;
+      continue;
+    }
+    ns2d_handle::AdjGraph* g = nullptr;
+    for (auto& c : h->adj_graph)
+      if (c.exec && c.dt == dt && c.C == C && c.wcur == h->wcur && c.lc == lc && c.lam == lam1 && c.tape == h->tape)
+        g = &c;
+    if (!g) {
+      g = !h->adj_graph[0].exec ? &h->adj_graph[0] : !h->adj_graph[1].exec ? &h->adj_graph[1] : &h->adj_graph[seg & 1];
+      if (g->exec) (void)hipGraphExecDestroy(g->exec);
+      g->exec = nullptr;
+      const int wcur0 = h->wcur, lc0 = lc;
+      HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      rc = segment(seg, cnt);
+      hipGraph_t gr = nullptr;
+      const hipError_t ec = hipStreamEndCapture(h->stream, &gr);
+      if (rc) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return rc;
+      }
+      if (ec != hipSuccess) return fail(NS2D_ERR_HIP, "adjoint graph capture: %s", hipGetErrorString(ec));
+      const hipError_t ei = hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      if (ei != hipSuccess) return fail(NS2D_ERR_HIP, "adjoint graph instantiate: %s", hipGetErrorString(ei));
+      g->dt = dt;
+      g->C = C;
+      g->wcur = wcur0;
+      g->lc = lc0;
+      g->lam = lam1;
+      g->tape = h->tape;
+      // the capture advanced the host-side parities; the replay does the work
+      h->wcur = wcur0;
+      lc = lc0;
+    }
+    HIPCHK(hipGraphLaunch(g->exec, h->stream));
+    h->wcur ^= (int)(cnt & 1);
+    lc ^= (int)(cnt & 1);
   }
   *res = h->lamp[lc];
   h->inv_valid = false;

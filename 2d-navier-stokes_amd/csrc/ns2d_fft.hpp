@@ -37,8 +37,11 @@ __device__ __forceinline__ double2 cmul_si(double2 a) {
   return SIGN < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
 }
 
-// padded LDS position of element j of a line
-__host__ __device__ constexpr int padx(int j) { return j + (j >> 3); }
+// padded LDS position of element j of a line: one slot of padding per 2^PS
+// elements (PS = 3 everywhere except column-pass shapes tuned otherwise)
+template <int PS = 3>
+__host__ __device__ constexpr int padx_s(int j) { return j + (j >> PS); }
+__host__ __device__ constexpr int padx(int j) { return padx_s<3>(j); }
 // LDS line stride for lines of length L: padded length, odd so that
 // neighbouring lines start on different banks
 __host__ __device__ constexpr int line_stride(int L) { return padx(L) + 1; }

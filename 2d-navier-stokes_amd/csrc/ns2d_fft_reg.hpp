@@ -11,8 +11,8 @@
 // transform starts and ends in registers: an N = 4096 line with E = 16 costs
 // three radix-16 passes and two LDS exchanges.
 //
-// LDS positions are padded (padx: one slot every 8 elements) so that the
-// stride-R writes of the first pass hit distinct banks.  Twiddles: one load
+// LDS positions are padded (padx_s<PS>: one slot every 2^PS elements) so that
+// the stride-R writes of the first pass hit distinct banks.  Twiddles: one load
 // w = tw[k*NS2D_TW_LEN/(P*R)] per butterfly from the host-built table
 // exp(-2*pi*i*m/NS2D_TW_LEN); the powers w^r are formed by multiplication
 // (apply_twiddles below).
@@ -107,7 +107,7 @@ __device__ __forceinline__ void apply_twiddles(double2* v, double2 w) {
   }
 }
 
-template <int N, int TL, int SIGN, bool BLOCK, int P>
+template <int N, int TL, int SIGN, bool BLOCK, int P, int PS>
 __device__ __forceinline__ void fft_reg_pass(double2* v, double2* buf, const double2* __restrict__ tw, int l) {
   constexpr int E = N / TL;
   constexpr int REM = N / P;
@@ -132,27 +132,28 @@ __device__ __forceinline__ void fft_reg_pass(double2* v, double2* buf, const dou
       } else {
         const int j0 = (i - k) * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) buf[padx(j0 + r * P)] = x[r];
+        for (int r = 0; r < R; ++r) buf[padx_s<PS>(j0 + r * P)] = x[r];
       }
     }
     if constexpr (!LAST) {
       xsync<BLOCK>();
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = buf[padx(l + TL * e)];
+      for (int e = 0; e < E; ++e) v[e] = buf[padx_s<PS>(l + TL * e)];
       xsync<BLOCK>();
-      fft_reg_pass<N, TL, SIGN, BLOCK, P * R>(v, buf, tw, l);
+      fft_reg_pass<N, TL, SIGN, BLOCK, P * R, PS>(v, buf, tw, l);
     }
   }
 }
 
 // In-register FFT of one line: v holds x[l + TL*e] on entry and X[l + TL*e]
-// on exit.  buf: this line's LDS exchange buffer (padx(N) elements).  All TL
-// threads of the line (and, with BLOCK, every thread of the block) must call.
-template <int N, int TL, int SIGN, bool BLOCK>
+// on exit.  buf: this line's LDS exchange buffer (padx_s<PS>(N) elements).
+// All TL threads of the line (and, with BLOCK, every thread of the block)
+// must call.
+template <int N, int TL, int SIGN, bool BLOCK, int PS = 3>
 __device__ __forceinline__ void fft_reg(double2* v, double2* buf, const double2* __restrict__ tw, int l) {
   static_assert((N & (N - 1)) == 0 && N >= 2 && N % TL == 0, "bad FFT shape");
   static_assert(N <= NS2D_TW_LEN, "twiddle table too short");
-  fft_reg_pass<N, TL, SIGN, BLOCK, 1>(v, buf, tw, l);
+  fft_reg_pass<N, TL, SIGN, BLOCK, 1, PS>(v, buf, tw, l);
 }
 
 }  // namespace ns2d

@@ -138,7 +138,8 @@ int kpass_variant_info(int nx, int idx, int* C, int* E, int* T, int* KS);
 hipError_t launch_rpass(int nx, int ny, int variant, const RPassArgs& a, hipStream_t s);
 int rpass_variant_count(int ny);
 int rpass_variant_info(int ny, int idx, int* TLR, int* E, int* T, int* PAIRS);
-int rpass_variant_min_waves(int ny, int idx);
+int rpass_variant_extra(int ny, int idx, int* W, int* PS, int* OFF);
+int kpass_variant_extra(int nx, int idx, int* W, int* PS, int* OFF);
 hipError_t launch_r2c_rows(int nx, int ny, const double* in, double2* out, const double2* tw, int ld,
                            hipStream_t s);
 hipError_t launch_measures(int nbins, const MeasArgs& a, hipStream_t s);

@@ -78,11 +78,14 @@ __device__ __forceinline__ double dealias_factor(const PhysArgs& ph, int kx, int
 // the integrator state (the new state goes to a separate buffer w_out, so no
 // workgroup reads what another writes).
 // ---------------------------------------------------------------------------
-template <int L, int C, int T, int KS>
+// PS: LDS padding (one slot per 2^PS elements); OFF: extra 16-B slots between
+// the columns' lines, which sets where the neighbouring column's accesses of
+// one wave fall among the LDS banks (adjacent lanes work on adjacent columns)
+template <int L, int C, int T, int KS, int PS, int OFF>
 __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
   constexpr int TL = T / C;        // threads per column
   constexpr int E = L / TL;        // rows per thread
-  constexpr int LS = line_stride(L);
+  constexpr int LS = padx_s<PS>(L) + OFF;
   constexpr int FPS = 4 / KS;      // inverse fields per split
   constexpr bool BLOCK = T > 64;
   static_assert(T % C == 0 && L % TL == 0 && E >= 2 && 4 % KS == 0, "bad kpass shape");
@@ -110,7 +113,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
       R[e] = colv ? a.nin[(size_t)x * ld + ky] : make_double2(0.0, 0.0);
       if ((e & 3) == 3) asm volatile("" ::: "memory");
     }
-    fft_reg<L, TL, -1, BLOCK>(R, buf, tw, opaque(l));
+    fft_reg<L, TL, -1, BLOCK, PS>(R, buf, tw, opaque(l));
     const double nrm = ph.norm2;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -280,7 +283,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
       else if (f == 2) F[e] = make_double2(-kx * s.y, kx * s.x);              // i kx w_hat
       else F[e] = s;                                                          // w_hat
     }
-    fft_reg<L, TL, +1, BLOCK>(F, buf, tw, l_o);
+    fft_reg<L, TL, +1, BLOCK, PS>(F, buf, tw, l_o);
     if (colv) {
       double2* __restrict__ gout = a.g[f];
 #pragma unroll
@@ -289,19 +292,19 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
   }
 }
 
-template <int L, int C, int T, int KS>
+template <int L, int C, int T, int KS, int PS, int OFF>
 __global__ __launch_bounds__(T) void kpass_kernel(KPassArgs a) {
-  kpass_body<L, C, T, KS>(a, xcd_remap((int)blockIdx.x, (int)gridDim.x));
+  kpass_body<L, C, T, KS, PS, OFF>(a, xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
 // All members of an ensemble size class in one launch: work items
 // [m * per_member, (m+1) * per_member) belong to member m, whose arguments
 // are tab[m] (uniform per workgroup, read with scalar loads).
-template <int L, int C, int T, int KS>
+template <int L, int C, int T, int KS, int PS, int OFF>
 __global__ __launch_bounds__(T) void kpass_batched(const KPassArgs* __restrict__ tab, int per_member) {
   const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   const int m = b / per_member;
-  kpass_body<L, C, T, KS>(tab[m], b - m * per_member);
+  kpass_body<L, C, T, KS, PS, OFF>(tab[m], b - m * per_member);
 }
 
 // ---------------------------------------------------------------------------
@@ -815,54 +818,65 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
 // T threads, KS workgroups per column group; the first listed for each L is
 // the default.  Rows per thread E = L*C/T.
 #define NS2D_KPASS_VARIANTS(X) \
-  X(8, 16, 64, 1)              \
-  X(16, 16, 64, 1)             \
-  X(32, 16, 64, 1)             \
-  X(64, 16, 64, 1)             \
-  X(128, 16, 128, 1)           \
-  X(128, 8, 64, 2)             \
-  X(128, 8, 128, 2)            \
-  X(256, 16, 256, 1)           \
-  X(256, 8, 128, 2)            \
-  X(256, 8, 128, 4)            \
-  X(256, 8, 256, 2)            \
-  X(512, 8, 256, 2)            \
-  X(512, 8, 256, 1)            \
-  X(512, 4, 128, 4)            \
-  X(512, 8, 256, 4)            \
-  X(512, 8, 512, 2)            \
-  X(512, 4, 256, 2)            \
-  X(1024, 2, 256, 2)           \
-  X(1024, 4, 256, 2)           \
-  X(1024, 4, 256, 1)           \
-  X(1024, 4, 256, 4)           \
-  X(1024, 2, 128, 2)           \
-  X(1024, 2, 128, 4)           \
-  X(1024, 1, 64, 4)            \
-  X(1024, 4, 512, 2)           \
-  X(1024, 4, 512, 1)           \
-  X(2048, 2, 256, 2)           \
-  X(2048, 2, 256, 1)           \
-  X(2048, 4, 512, 1)           \
-  X(2048, 1, 128, 4)           \
-  X(2048, 2, 512, 1)           \
-  X(2048, 4, 1024, 1)          \
-  X(4096, 2, 512, 1)           \
-  X(4096, 1, 256, 1)           \
-  X(4096, 1, 256, 2)           \
-  X(4096, 2, 512, 2)           \
-  X(4096, 2, 1024, 1)          \
-  X(4096, 1, 512, 1)           \
-  X(4096, 1, 512, 2)
+  X(8, 16, 64, 1, 3, 1)            \
+  X(16, 16, 64, 1, 3, 1)           \
+  X(32, 16, 64, 1, 3, 1)           \
+  X(64, 16, 64, 1, 3, 1)           \
+  X(128, 16, 128, 1, 3, 1)         \
+  X(128, 8, 64, 2, 3, 1)           \
+  X(128, 8, 128, 2, 3, 1)          \
+  X(256, 16, 256, 1, 3, 1)         \
+  X(256, 8, 128, 2, 3, 1)          \
+  X(256, 8, 128, 4, 3, 1)          \
+  X(256, 8, 256, 2, 3, 1)          \
+  X(512, 8, 256, 2, 3, 1)          \
+  X(512, 8, 256, 1, 3, 1)          \
+  X(512, 4, 128, 4, 3, 1)          \
+  X(512, 8, 256, 4, 3, 1)          \
+  X(512, 8, 512, 2, 3, 1)          \
+  X(512, 4, 256, 2, 3, 1)          \
+  X(1024, 2, 256, 2, 3, 1)         \
+  X(1024, 2, 256, 2, 3, 4)         \
+  X(1024, 2, 256, 2, 3, 8)         \
+  X(1024, 4, 256, 2, 3, 1)         \
+  X(1024, 4, 256, 1, 3, 1)         \
+  X(1024, 4, 256, 4, 3, 1)         \
+  X(1024, 2, 128, 2, 3, 1)         \
+  X(1024, 2, 128, 4, 3, 1)         \
+  X(1024, 1, 64, 4, 3, 1)          \
+  X(1024, 4, 512, 2, 3, 1)         \
+  X(1024, 4, 512, 1, 3, 1)         \
+  X(1024, 8, 512, 4, 4, 4)         \
+  X(1024, 8, 1024, 4, 3, 4)        \
+  X(1024, 8, 1024, 2, 3, 4)        \
+  X(2048, 2, 256, 2, 3, 1)         \
+  X(2048, 2, 256, 1, 3, 1)         \
+  X(2048, 4, 512, 1, 3, 1)         \
+  X(2048, 1, 128, 4, 3, 1)         \
+  X(2048, 2, 512, 1, 3, 1)         \
+  X(2048, 4, 1024, 1, 3, 1)        \
+  X(4096, 2, 512, 1, 3, 1)         \
+  X(4096, 2, 512, 1, 4, 4)         \
+  X(4096, 2, 512, 1, 4, 8)         \
+  X(4096, 2, 512, 1, 3, 4)         \
+  X(4096, 1, 256, 1, 3, 1)         \
+  X(4096, 1, 256, 2, 3, 1)         \
+  X(4096, 2, 512, 2, 3, 1)         \
+  X(4096, 2, 1024, 1, 3, 1)        \
+  X(4096, 2, 1024, 1, 3, 4)        \
+  X(4096, 2, 1024, 1, 3, 8)        \
+  X(4096, 1, 512, 1, 3, 1)         \
+  X(4096, 1, 512, 2, 3, 1)
 
 struct KVariant {
-  int L, C, T, KS;
+  int L, C, T, KS, PS, OFF;
   void (*kern)(KPassArgs);
   void (*batched)(const KPassArgs*, int);
 };
 
-#define NS2D_KV_ENTRY(L_, C_, T_, KS_) \
-  {L_, C_, T_, KS_, kpass_kernel<L_, C_, T_, KS_>, kpass_batched<L_, C_, T_, KS_>},
+#define NS2D_KV_ENTRY(L_, C_, T_, KS_, PS_, OFF_)                                    \
+  {L_,  C_,  T_, KS_, PS_, OFF_, kpass_kernel<L_, C_, T_, KS_, PS_, OFF_>, \
+   kpass_batched<L_, C_, T_, KS_, PS_, OFF_>},
 static const KVariant kvariants[] = {NS2D_KPASS_VARIANTS(NS2D_KV_ENTRY)};
 #undef NS2D_KV_ENTRY
 
@@ -901,6 +915,15 @@ int kpass_variant_info(int nx, int idx, int* C, int* E, int* T, int* KS) {
   return 0;
 }
 
+int kpass_variant_extra(int nx, int idx, int* W, int* PS, int* OFF) {
+  const KVariant* v = find_kvariant(nx, idx);
+  if (!v) return -1;
+  *W = 1;
+  *PS = v->PS;
+  *OFF = v->OFF;
+  return 0;
+}
+
 // Launch the kpass for an nx-point column transform.  A pass without inverse
 // transforms (an RHS evaluation) uses a KS = 1 sibling of the same shape:
 // split workgroups exist only to share the inverse fields.
@@ -910,7 +933,10 @@ hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s) 
   if (!(a.mode & KP_INV) && v->KS != 1) {
     const KVariant* w = nullptr;
     for (const KVariant& u : kvariants)
-      if (u.L == nx && u.KS == 1 && u.C == v->C && u.T == v->T) { w = &u; break; }
+      if (u.L == nx && u.KS == 1 && u.C == v->C && u.T == v->T && u.PS == v->PS && u.OFF == v->OFF) {
+        w = &u;
+        break;
+      }
     if (!w)
       for (const KVariant& u : kvariants)
         if (u.L == nx && u.KS == 1) { w = &u; break; }
@@ -998,9 +1024,13 @@ int rpass_variant_info(int ny, int idx, int* TLR, int* E, int* T, int* PAIRS) {
   return 0;
 }
 
-int rpass_variant_min_waves(int ny, int idx) {
+int rpass_variant_extra(int ny, int idx, int* W, int* PS, int* OFF) {
   const RVariant* v = find_rvariant(ny, idx);
-  return v ? v->W : -1;
+  if (!v) return -1;
+  *W = v->W;
+  *PS = 3;
+  *OFF = 1;
+  return 0;
 }
 
 hipError_t launch_rpass(int nx, int ny, int variant, const RPassArgs& a, hipStream_t s) {

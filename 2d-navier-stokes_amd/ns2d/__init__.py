@@ -63,7 +63,7 @@ class _Info(ctypes.Structure):
 EXPORTS = ["ns2d_params_default", "ns2d_create", "ns2d_destroy", "ns2d_set_state", "ns2d_get_state",
            "ns2d_set_state_device", "ns2d_get_state_device", "ns2d_step", "ns2d_rhs", "ns2d_measures",
            "ns2d_max_vort", "ns2d_real_vort", "ns2d_r2c", "ns2d_c2r", "ns2d_get_info", "ns2d_sync",
-           "ns2d_get_stream", "ns2d_profile_steps", "ns2d_variant_count", "ns2d_variant_info", "ns2d_variant_min_waves",
+           "ns2d_get_stream", "ns2d_profile_steps", "ns2d_variant_count", "ns2d_variant_info", "ns2d_variant_extra",
            "ns2d_set_variant", "ns2d_ens_create", "ns2d_ens_member", "ns2d_ens_step", "ns2d_ens_sync",
            "ns2d_ens_destroy", "ns2d_vjp", "ns2d_vjp_device", "ns2d_last_error"]
 
@@ -117,7 +117,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ns2d_profile_steps": ([vp, d, l, dp, dp, dp], i),
         "ns2d_variant_count": ([i, i], i),
         "ns2d_variant_info": ([i, i, i, P(i)], i),
-        "ns2d_variant_min_waves": ([i, i, i], i),
+        "ns2d_variant_extra": ([i, i, i, P(i)], i),
         "ns2d_set_variant": ([vp, i, i], i),
         "ns2d_ens_create": ([P(_Params), i, P(vp)], i),
         "ns2d_ens_member": ([vp, i, P(vp)], i),
@@ -151,7 +151,9 @@ def variants(pass_kind: int, n: int) -> list:
         v = (ctypes.c_int * 4)()
         _check(lib.ns2d_variant_info(pass_kind, n, i, v))
         d = dict(zip(keys, list(v)))
-        d["min_waves"] = int(lib.ns2d_variant_min_waves(pass_kind, n, i))
+        x = (ctypes.c_int * 4)()
+        _check(lib.ns2d_variant_extra(pass_kind, n, i, x))
+        d.update(min_waves=x[0], pad_shift=x[1], line_offset=x[2])
         out.append(d)
     return out
 

@@ -167,9 +167,10 @@ int ns2d_profile_steps(ns2d_t* h, double dt, long nsteps, double* kpass_ms, doub
 #define NS2D_PASS_ROW 1
 int ns2d_variant_count(int pass, int n);
 int ns2d_variant_info(int pass, int n, int idx, int* shape4);
-/* The register budget a shape was compiled for, as the minimum number of
- * waves per SIMD it must allow (1: no constraint). */
-int ns2d_variant_min_waves(int pass, int n, int idx);
+/* Further compile-time choices of a shape: extra4 = {minimum waves per SIMD
+ * its register budget allows (1: no constraint), LDS padding shift (one slot
+ * per 2^shift elements), 16-B slots between neighbouring lines, 0}. */
+int ns2d_variant_extra(int pass, int n, int idx, int* extra4);
 int ns2d_set_variant(ns2d_t* h, int pass, int idx);
 
 /* Ensemble (no counterpart in the reference, which runs one realisation per

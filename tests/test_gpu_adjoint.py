@@ -116,6 +116,28 @@ def test_vjp_after_forward_steps_on_the_same_handle():
     assert np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref) < 1e-11
 
 
+def test_vjp_graph_replayed_segments(monkeypatch):
+    """Eight checkpoint segments: the middle ones run as replayed hipGraphs
+    (one per buffer parity); equal to the direct launches bit for bit and to
+    torch autograd."""
+    p = O.Params(64, 64, nu=1e-3, hyper_flag=0, hyper_pow=1.0, quirks=0)
+    w0 = O.ic_random_spectrum(p, seed=12, k0=5.0)
+    dt, n = 2e-3, 30
+    J, g_ref, wT = torch_grad(p, w0, dt, n)
+    lam = enstrophy_grad(p, wT)
+    with make(p) as s:
+        g1 = s.vjp(w0, dt, n, lam, ckpt_every=4)
+        g1b = s.vjp(w0, dt, n, lam, ckpt_every=4)      # cached graphs
+        g3 = s.vjp(w0, dt, n, lam, ckpt_every=3)       # odd segment length
+    monkeypatch.setenv("NS2D_GRAPH", "0")
+    with make(p) as s:
+        g2 = s.vjp(w0, dt, n, lam, ckpt_every=4)
+    np.testing.assert_array_equal(g1, g2)
+    np.testing.assert_array_equal(g1, g1b)
+    assert np.linalg.norm(g1 - g_ref) / np.linalg.norm(g_ref) < 1e-11
+    assert np.linalg.norm(g3 - g_ref) / np.linalg.norm(g_ref) < 1e-11
+
+
 def test_vjp_rejects_reference_conjugacy_quirk():
     import ns2d
     p = O.Params(32, 32, nu=1e-3)
