@@ -29,6 +29,8 @@
 #include "ns2d_fft.hpp"
 #include "ns2d_fft_reg.hpp"
 #include "ns2d_internal.h"
+#include <cstdio>
+#include <cstdlib>
 
 namespace ns2d {
 
@@ -818,55 +820,55 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
 // T threads, KS workgroups per column group; the first listed for each L is
 // the default.  Rows per thread E = L*C/T.
 #define NS2D_KPASS_VARIANTS(X) \
-  X(8, 16, 64, 1, 3, 1)            \
-  X(16, 16, 64, 1, 3, 1)           \
-  X(32, 16, 64, 1, 3, 1)           \
-  X(64, 16, 64, 1, 3, 1)           \
-  X(128, 16, 128, 1, 3, 1)         \
-  X(128, 8, 64, 2, 3, 1)           \
-  X(128, 8, 128, 2, 3, 1)          \
-  X(256, 16, 256, 1, 3, 1)         \
-  X(256, 8, 128, 2, 3, 1)          \
-  X(256, 8, 128, 4, 3, 1)          \
-  X(256, 8, 256, 2, 3, 1)          \
-  X(512, 8, 256, 2, 3, 1)          \
-  X(512, 8, 256, 1, 3, 1)          \
-  X(512, 4, 128, 4, 3, 1)          \
-  X(512, 8, 256, 4, 3, 1)          \
-  X(512, 8, 512, 2, 3, 1)          \
-  X(512, 4, 256, 2, 3, 1)          \
-  X(1024, 2, 256, 2, 3, 1)         \
-  X(1024, 2, 256, 2, 3, 4)         \
-  X(1024, 2, 256, 2, 3, 8)         \
-  X(1024, 4, 256, 2, 3, 1)         \
-  X(1024, 4, 256, 1, 3, 1)         \
-  X(1024, 4, 256, 4, 3, 1)         \
-  X(1024, 2, 128, 2, 3, 1)         \
-  X(1024, 2, 128, 4, 3, 1)         \
-  X(1024, 1, 64, 4, 3, 1)          \
-  X(1024, 4, 512, 2, 3, 1)         \
-  X(1024, 4, 512, 1, 3, 1)         \
-  X(1024, 8, 512, 4, 4, 4)         \
-  X(1024, 8, 1024, 4, 3, 4)        \
-  X(1024, 8, 1024, 2, 3, 4)        \
-  X(2048, 2, 256, 2, 3, 1)         \
-  X(2048, 2, 256, 1, 3, 1)         \
-  X(2048, 4, 512, 1, 3, 1)         \
-  X(2048, 1, 128, 4, 3, 1)         \
-  X(2048, 2, 512, 1, 3, 1)         \
-  X(2048, 4, 1024, 1, 3, 1)        \
-  X(4096, 2, 512, 1, 3, 1)         \
-  X(4096, 2, 512, 1, 4, 4)         \
-  X(4096, 2, 512, 1, 4, 8)         \
-  X(4096, 2, 512, 1, 3, 4)         \
-  X(4096, 1, 256, 1, 3, 1)         \
-  X(4096, 1, 256, 2, 3, 1)         \
-  X(4096, 2, 512, 2, 3, 1)         \
-  X(4096, 2, 1024, 1, 3, 1)        \
-  X(4096, 2, 1024, 1, 3, 4)        \
-  X(4096, 2, 1024, 1, 3, 8)        \
-  X(4096, 1, 512, 1, 3, 1)         \
-  X(4096, 1, 512, 2, 3, 1)
+  X(8, 16, 64, 1, 3, 1)       \
+  X(16, 16, 64, 1, 3, 1)      \
+  X(32, 16, 64, 1, 3, 1)      \
+  X(64, 16, 64, 1, 3, 1)      \
+  X(128, 16, 128, 1, 3, 1)    \
+  X(128, 8, 64, 2, 3, 1)      \
+  X(128, 8, 128, 2, 3, 1)     \
+  X(256, 8, 256, 2, 3, 1)     \
+  X(256, 16, 256, 1, 3, 1)    \
+  X(256, 8, 128, 2, 3, 1)     \
+  X(256, 8, 128, 4, 3, 1)     \
+  X(512, 4, 256, 2, 3, 1)     \
+  X(512, 8, 256, 2, 3, 1)     \
+  X(512, 8, 256, 1, 3, 1)     \
+  X(512, 4, 128, 4, 3, 1)     \
+  X(512, 8, 256, 4, 3, 1)     \
+  X(512, 8, 512, 2, 3, 1)     \
+  X(1024, 2, 256, 2, 3, 1)    \
+  X(1024, 2, 256, 2, 3, 4)    \
+  X(1024, 2, 256, 2, 3, 8)    \
+  X(1024, 4, 256, 2, 3, 1)    \
+  X(1024, 4, 256, 1, 3, 1)    \
+  X(1024, 4, 256, 4, 3, 1)    \
+  X(1024, 2, 128, 2, 3, 1)    \
+  X(1024, 2, 128, 4, 3, 1)    \
+  X(1024, 1, 64, 4, 3, 1)     \
+  X(1024, 4, 512, 2, 3, 1)    \
+  X(1024, 4, 512, 1, 3, 1)    \
+  X(1024, 8, 512, 4, 4, 4)    \
+  X(1024, 8, 1024, 4, 3, 4)   \
+  X(1024, 8, 1024, 2, 3, 4)   \
+  X(2048, 2, 512, 1, 3, 1)    \
+  X(2048, 2, 256, 2, 3, 1)    \
+  X(2048, 2, 256, 1, 3, 1)    \
+  X(2048, 4, 512, 1, 3, 1)    \
+  X(2048, 1, 128, 4, 3, 1)    \
+  X(2048, 4, 1024, 1, 3, 1)   \
+  X(4096, 2, 512, 1, 3, 1)    \
+  X(4096, 2, 512, 1, 4, 4)    \
+  X(4096, 2, 512, 1, 4, 8)    \
+  X(4096, 2, 512, 1, 3, 4)    \
+  X(4096, 1, 256, 1, 3, 1)    \
+  X(4096, 1, 256, 2, 3, 1)    \
+  X(4096, 2, 512, 2, 3, 1)    \
+  X(4096, 2, 1024, 1, 3, 1)   \
+  X(4096, 2, 1024, 1, 3, 4)   \
+  X(4096, 2, 1024, 1, 3, 8)   \
+  X(4096, 1, 512, 1, 3, 1)    \
+  X(4096, 1, 512, 2, 3, 1)   
 
 struct KVariant {
   int L, C, T, KS, PS, OFF;
@@ -952,33 +954,33 @@ hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s) 
 // per thread), PAIRS row pairs per workgroup; the first listed for each NY is
 // the default.
 #define NS2D_RPASS_VARIANTS(X) \
-  X(8, 1, 64, 1)               \
-  X(16, 1, 64, 1)              \
-  X(32, 2, 32, 1)              \
-  X(32, 4, 16, 1)              \
-  X(64, 4, 16, 1)              \
-  X(64, 8, 8, 1)               \
-  X(128, 8, 8, 1)              \
-  X(128, 16, 4, 1)             \
-  X(128, 32, 2, 1)             \
-  X(256, 16, 4, 1)             \
-  X(256, 32, 2, 1)             \
-  X(256, 64, 1, 1)             \
-  X(512, 32, 2, 1)             \
-  X(512, 64, 1, 1)             \
-  X(512, 128, 1, 1)            \
-  X(1024, 256, 1, 1)           \
-  X(1024, 64, 1, 1)            \
-  X(1024, 128, 1, 1)           \
-  X(2048, 128, 1, 1)           \
-  X(2048, 128, 1, 2)           \
-  X(2048, 256, 1, 1)           \
-  X(2048, 512, 1, 1)           \
-  X(4096, 512, 1, 4)           \
-  X(4096, 256, 1, 1)           \
-  X(4096, 256, 1, 2)           \
-  X(4096, 512, 1, 1)           \
-  X(4096, 1024, 1, 1)
+  X(8, 1, 64, 1)        \
+  X(16, 1, 64, 1)       \
+  X(32, 2, 32, 1)       \
+  X(32, 4, 16, 1)       \
+  X(64, 4, 16, 1)       \
+  X(64, 8, 8, 1)        \
+  X(128, 8, 8, 1)       \
+  X(128, 16, 4, 1)      \
+  X(128, 32, 2, 1)      \
+  X(256, 64, 1, 1)      \
+  X(256, 16, 4, 1)      \
+  X(256, 32, 2, 1)      \
+  X(512, 64, 1, 1)      \
+  X(512, 32, 2, 1)      \
+  X(512, 128, 1, 1)     \
+  X(1024, 256, 1, 1)    \
+  X(1024, 64, 1, 1)     \
+  X(1024, 128, 1, 1)    \
+  X(2048, 128, 1, 2)    \
+  X(2048, 128, 1, 1)    \
+  X(2048, 256, 1, 1)    \
+  X(2048, 512, 1, 1)    \
+  X(4096, 512, 1, 4)    \
+  X(4096, 256, 1, 1)    \
+  X(4096, 256, 1, 2)    \
+  X(4096, 512, 1, 1)    \
+  X(4096, 1024, 1, 1)  
 
 struct RVariant {
   int NY, TLR, PAIRS, W;
@@ -1076,53 +1078,114 @@ hipError_t launch_measures(int nbins, const MeasArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// adjoint pass shapes (one per transform length): the default forward shapes
-template <int NY>
-static hipError_t launch_rpass_adj_t(int nx, const RAdjArgs& a, hipStream_t s) {
-  constexpr int E = NY >= 16 ? 16 : NY;
-  constexpr int TLR = NY / E;
-  constexpr int PAIRS = TLR >= 64 ? 1 : 64 / TLR;
-  hipLaunchKernelGGL((rpass_adj_kernel<NY, TLR, PAIRS>), dim3((nx / 2 + PAIRS - 1) / PAIRS), dim3(TLR * PAIRS), 0,
-                     s, a);
-  return hipGetLastError();
+// adjoint pass shapes per transform length; index 0 is the default, others
+// are selectable (NS2D_KADJ_VARIANT_<N>, NS2D_RADJ_VARIANT_<N>) for tuning
+#define NS2D_RADJ_VARIANTS(X) \
+  X(8, 1, 64)       \
+  X(16, 1, 64)      \
+  X(32, 4, 16)      \
+  X(64, 8, 8)       \
+  X(128, 32, 2)     \
+  X(128, 16, 4)     \
+  X(256, 64, 1)     \
+  X(256, 16, 4)     \
+  X(256, 32, 2)     \
+  X(512, 64, 1)     \
+  X(512, 128, 1)    \
+  X(512, 32, 2)     \
+  X(1024, 256, 1)   \
+  X(1024, 64, 1)    \
+  X(1024, 128, 1)   \
+  X(2048, 256, 1)   \
+  X(2048, 128, 1)   \
+  X(2048, 512, 1)   \
+  X(4096, 512, 1)   \
+  X(4096, 256, 1)  
+
+#define NS2D_KADJ_VARIANTS(X) \
+  X(8, 16, 64)       \
+  X(16, 16, 64)      \
+  X(32, 16, 64)      \
+  X(64, 16, 64)      \
+  X(128, 16, 128)    \
+  X(128, 8, 128)     \
+  X(256, 4, 256)     \
+  X(256, 4, 128)     \
+  X(256, 8, 128)     \
+  X(512, 2, 256)     \
+  X(512, 2, 128)     \
+  X(512, 4, 128)     \
+  X(512, 4, 256)     \
+  X(1024, 2, 512)    \
+  X(1024, 2, 256)    \
+  X(1024, 2, 128)    \
+  X(2048, 2, 512)    \
+  X(2048, 2, 256)    \
+  X(4096, 2, 1024)   \
+  X(4096, 1, 256)    \
+  X(4096, 2, 512)   
+
+struct RAdjVariant {
+  int NY, TLR, PAIRS;
+  void (*kern)(RAdjArgs);
+};
+struct KAdjVariant {
+  int L, C, T;
+  void (*kern)(KAdjArgs);
+};
+#define NS2D_RADJ_ENTRY(NY_, TLR_, PAIRS_) {NY_, TLR_, PAIRS_, rpass_adj_kernel<NY_, TLR_, PAIRS_>},
+#define NS2D_KADJ_ENTRY(L_, C_, T_) {L_, C_, T_, kpass_adj_kernel<L_, C_, T_>},
+static const RAdjVariant radj_variants[] = {NS2D_RADJ_VARIANTS(NS2D_RADJ_ENTRY)};
+static const KAdjVariant kadj_variants[] = {NS2D_KADJ_VARIANTS(NS2D_KADJ_ENTRY)};
+#undef NS2D_RADJ_ENTRY
+#undef NS2D_KADJ_ENTRY
+
+// index chosen through the environment (read once per length), default 0
+static int adj_choice(const char* prefix, int n) {
+  char name[64];
+  std::snprintf(name, sizeof name, "%s%d", prefix, n);
+  const char* v = std::getenv(name);
+  return v ? std::atoi(v) : 0;
 }
 
 hipError_t launch_rpass_adj(int nx, int ny, const RAdjArgs& a, hipStream_t s) {
-  switch (ny) {
-    case 8: return launch_rpass_adj_t<8>(nx, a, s);
-    case 16: return launch_rpass_adj_t<16>(nx, a, s);
-    case 32: return launch_rpass_adj_t<32>(nx, a, s);
-    case 64: return launch_rpass_adj_t<64>(nx, a, s);
-    case 128: return launch_rpass_adj_t<128>(nx, a, s);
-    case 256: return launch_rpass_adj_t<256>(nx, a, s);
-    case 512: return launch_rpass_adj_t<512>(nx, a, s);
-    case 1024: return launch_rpass_adj_t<1024>(nx, a, s);
-    case 2048: return launch_rpass_adj_t<2048>(nx, a, s);
-    case 4096: return launch_rpass_adj_t<4096>(nx, a, s);
-    default: return hipErrorInvalidValue;
+  static int idx[16] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  const int lg = ny > 0 ? 31 - __builtin_clz((unsigned)ny) : 0;
+  if (lg >= 16) return hipErrorInvalidValue;
+  if (idx[lg] < 0) idx[lg] = adj_choice("NS2D_RADJ_VARIANT_", ny);
+  const RAdjVariant* v = nullptr;
+  int k = 0;
+  for (const RAdjVariant& u : radj_variants) {
+    if (u.NY != ny) continue;
+    if (!v) v = &u;
+    if (k++ == idx[lg]) {
+      v = &u;
+      break;
+    }
   }
-}
-
-template <int L, int C, int T>
-static hipError_t launch_kpass_adj_t(const KAdjArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((kpass_adj_kernel<L, C, T>), dim3((a.nyf + C - 1) / C), dim3(T), 0, s, a);
+  if (!v) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(v->kern, dim3((nx / 2 + v->PAIRS - 1) / v->PAIRS), dim3(v->TLR * v->PAIRS), 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_kpass_adj(int nx, const KAdjArgs& a, hipStream_t s) {
-  switch (nx) {
-    case 8: return launch_kpass_adj_t<8, 16, 64>(a, s);
-    case 16: return launch_kpass_adj_t<16, 16, 64>(a, s);
-    case 32: return launch_kpass_adj_t<32, 16, 64>(a, s);
-    case 64: return launch_kpass_adj_t<64, 16, 64>(a, s);
-    case 128: return launch_kpass_adj_t<128, 16, 128>(a, s);
-    case 256: return launch_kpass_adj_t<256, 8, 128>(a, s);
-    case 512: return launch_kpass_adj_t<512, 4, 128>(a, s);
-    case 1024: return launch_kpass_adj_t<1024, 2, 128>(a, s);
-    case 2048: return launch_kpass_adj_t<2048, 2, 256>(a, s);
-    case 4096: return launch_kpass_adj_t<4096, 1, 256>(a, s);
-    default: return hipErrorInvalidValue;
+  static int idx[16] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  const int lg = nx > 0 ? 31 - __builtin_clz((unsigned)nx) : 0;
+  if (lg >= 16) return hipErrorInvalidValue;
+  if (idx[lg] < 0) idx[lg] = adj_choice("NS2D_KADJ_VARIANT_", nx);
+  const KAdjVariant* v = nullptr;
+  int k = 0;
+  for (const KAdjVariant& u : kadj_variants) {
+    if (u.L != nx) continue;
+    if (!v) v = &u;
+    if (k++ == idx[lg]) {
+      v = &u;
+      break;
+    }
   }
+  if (!v) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(v->kern, dim3((a.nyf + v->C - 1) / v->C), dim3(v->T), 0, s, a);
+  return hipGetLastError();
 }
 
 }  // namespace ns2d
