@@ -961,8 +961,12 @@ struct ns2d_ensemble {
     std::vector<int> idx;
     KPassArgs* dk = nullptr;   // [parity 2][stage 4][members]
     RPassArgs* dr = nullptr;   // [stage 4][members]
+    hipStream_t stream = nullptr;  // the class's launches run concurrently
+    hipEvent_t done = nullptr;
   };
   std::vector<Class> cls;
+  hipEvent_t fork = nullptr;
+  bool tuned = false;
   bool tables_ok = false;
   int tables_parity = -1;
   bool tables_check = false;
@@ -1020,19 +1024,97 @@ int ens_tables(ns2d_ensemble* e, bool check) {
 }
 
 // One batched RK4CN step of every member; p2 = parity relative to the tables.
+// Size classes are independent: each runs its stages on its own stream,
+// forked from and joined back into the ensemble stream (also under capture,
+// where the fork/join becomes parallel branches of the graph).
 int ens_launch_step(ns2d_ensemble* e, int p2) {
-  for (int st = 0; st < 4; ++st) {
-    for (auto& c : e->cls) {
-      hipError_t r = launch_rpass_batched(c.nx, c.ny, c.rvar, c.dr + (size_t)st * c.idx.size(), (int)c.idx.size(),
-                                          e->stream);
+  const bool fan = e->cls.size() > 1;
+  if (fan) HIPCHK(hipEventRecord(e->fork, e->stream));
+  for (auto& c : e->cls) {
+    hipStream_t cs = fan ? c.stream : e->stream;
+    if (fan) HIPCHK(hipStreamWaitEvent(cs, e->fork, 0));
+    for (int st = 0; st < 4; ++st) {
+      hipError_t r = launch_rpass_batched(c.nx, c.ny, c.rvar, c.dr + (size_t)st * c.idx.size(), (int)c.idx.size(), cs);
       if (r != hipSuccess) return fail(NS2D_ERR_HIP, "batched rpass: %s", hipGetErrorString(r));
-    }
-    for (auto& c : e->cls) {
-      hipError_t r = launch_kpass_batched(c.nx, c.nyf, c.kvar, c.dk + ((size_t)p2 * 4 + st) * c.idx.size(),
-                                          (int)c.idx.size(), e->stream);
+      r = launch_kpass_batched(c.nx, c.nyf, c.kvar, c.dk + ((size_t)p2 * 4 + st) * c.idx.size(), (int)c.idx.size(),
+                               cs);
       if (r != hipSuccess) return fail(NS2D_ERR_HIP, "batched kpass: %s", hipGetErrorString(r));
     }
+    if (fan) {
+      HIPCHK(hipEventRecord(c.done, cs));
+      HIPCHK(hipStreamWaitEvent(e->stream, c.done, 0));
+    }
   }
+  return NS2D_OK;
+}
+
+// The project's autotuner for the batched passes: per size class, time the
+// stage-1 row pass and column pass of every compiled shape over all members
+// of the class (HIP events, best of `reps`) and keep the fastest.  The column
+// pass of stage 1 overwrites the members' transformed fields, which are
+// rebuilt from their states afterwards; the states themselves are not touched.
+int ens_autotune(ns2d_ensemble* e, int reps) {
+  int rc = ens_tables(e, e->m[0]->p.check_every != 0);
+  if (rc) return rc;
+  // the step-check words (iteration count, blow-up record) survive the timing
+  std::vector<unsigned long long> saved((size_t)STAT_N * e->m.size());
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    rc = read_stat(e->m[i], &saved[i * STAT_N]);
+    if (rc) return rc;
+  }
+  hipEvent_t t0, t1;
+  HIPCHK(hipEventCreate(&t0));
+  HIPCHK(hipEventCreate(&t1));
+  auto time_it = [&](auto&& launch) -> float {
+    float best = 1e30f;
+    for (int r = 0; r < reps; ++r) {
+      if (hipEventRecord(t0, e->stream) != hipSuccess) return -1.0f;
+      if (launch() != hipSuccess) return -1.0f;
+      if (hipEventRecord(t1, e->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess) return -1.0f;
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, t0, t1) != hipSuccess) return -1.0f;
+      best = ms < best ? ms : best;
+    }
+    return best;
+  };
+  for (auto& c : e->cls) {
+    const int nm = (int)c.idx.size();
+    float best = 1e30f;
+    int arg = c.rvar;
+    for (int v = 0; v < rpass_variant_count(c.ny); ++v) {
+      const float ms = time_it([&] { return launch_rpass_batched(c.nx, c.ny, v, c.dr, nm, e->stream); });
+      if (ms >= 0.0f && ms < best) {
+        best = ms;
+        arg = v;
+      }
+    }
+    c.rvar = arg;
+    best = 1e30f;
+    arg = c.kvar;
+    for (int v = 0; v < kpass_variant_count(c.nx); ++v) {
+      const float ms = time_it([&] { return launch_kpass_batched(c.nx, c.nyf, v, c.dk, nm, e->stream); });
+      if (ms >= 0.0f && ms < best) {
+        best = ms;
+        arg = v;
+      }
+    }
+    c.kvar = arg;
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  HIPCHK(hipGetLastError());
+  // rebuild what the timing runs overwrote
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    ns2d_handle* h = e->m[i];
+    h->inv_valid = false;
+    rc = prepare_inv(h);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(h->stat, &saved[i * STAT_N], sizeof(unsigned long long) * STAT_N, hipMemcpyHostToDevice,
+                          h->stream));
+  }
+  HIPCHK(hipStreamSynchronize(e->stream));
+  ens_drop(e);
+  e->tuned = true;
   return NS2D_OK;
 }
 
@@ -1056,7 +1138,9 @@ int ns2d_ens_create(const ns2d_params* members, int n, ns2d_ens_t** out) {
       return fail(NS2D_ERR_INVALID, "ensemble member %d: only RK4CN members are batched", i);
   HIPCHK(hipSetDevice(members[0].device));
   ns2d_ensemble* e = new ns2d_ensemble();
-  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess) {
+    if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return fail(NS2D_ERR_HIP, "stream create failed");
   }
@@ -1079,6 +1163,12 @@ int ns2d_ens_create(const ns2d_params* members, int n, ns2d_ens_t** out) {
       c.nyf = h->nyf;
       c.kvar = h->kvar;
       c.rvar = h->rvar;
+      if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&c.done, hipEventDisableTiming) != hipSuccess) {
+        e->cls.push_back(c);
+        ns2d_ens_destroy(e);
+        return fail(NS2D_ERR_HIP, "ensemble stream create failed");
+      }
       e->cls.push_back(c);
       ci = (int)e->cls.size() - 1;
     }
@@ -1124,6 +1214,14 @@ int ns2d_ens_step(ns2d_ens_t* e, const double* dt, long nsteps) {
     }
     e->tables_ok = false;
     ++s;
+  }
+  if (s < nsteps && !e->tuned) {
+    const char* t = std::getenv("NS2D_ENS_TUNE");
+    if (!t || std::atoi(t) != 0) {
+      int rc = ens_autotune(e, 3);
+      if (rc) return rc;
+    }
+    e->tuned = true;
   }
   if (s < nsteps) {
     int rc = ens_tables(e, check);
@@ -1185,6 +1283,28 @@ int ns2d_ens_step(ns2d_ens_t* e, const double* dt, long nsteps) {
   return NS2D_OK;
 }
 
+int ns2d_ens_autotune(ns2d_ens_t* e, int reps) {
+  if (!e || reps < 1) return fail(NS2D_ERR_INVALID, "bad autotune arguments");
+  HIPCHK(hipSetDevice(e->m[0]->p.device));
+  for (ns2d_handle* h : e->m) {
+    int rc = prepare_inv(h);
+    if (rc) return rc;
+  }
+  return ens_autotune(e, reps);
+}
+
+int ns2d_ens_variants(ns2d_ens_t* e, int member, int* kpass_idx, int* rpass_idx) {
+  if (!e || !kpass_idx || !rpass_idx || member < 0 || member >= (int)e->m.size())
+    return fail(NS2D_ERR_INVALID, "bad arguments");
+  for (auto& c : e->cls)
+    for (int i : c.idx)
+      if (i == member) {
+        *kpass_idx = c.kvar;
+        *rpass_idx = c.rvar;
+      }
+  return NS2D_OK;
+}
+
 int ns2d_ens_sync(ns2d_ens_t* e) {
   if (!e) return fail(NS2D_ERR_INVALID, "null ensemble");
   HIPCHK(hipStreamSynchronize(e->stream));
@@ -1196,10 +1316,14 @@ int ns2d_ens_destroy(ns2d_ens_t* e) {
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   ens_drop(e);
   for (auto& c : e->cls) {
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
     if (c.dk) (void)hipFree(c.dk);
     if (c.dr) (void)hipFree(c.dr);
+    if (c.done) (void)hipEventDestroy(c.done);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
   }
   for (ns2d_handle* h : e->m) ns2d_destroy(h);
+  if (e->fork) (void)hipEventDestroy(e->fork);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return NS2D_OK;

@@ -64,7 +64,7 @@ EXPORTS = ["ns2d_params_default", "ns2d_create", "ns2d_destroy", "ns2d_set_state
            "ns2d_set_state_device", "ns2d_get_state_device", "ns2d_step", "ns2d_rhs", "ns2d_measures",
            "ns2d_max_vort", "ns2d_real_vort", "ns2d_r2c", "ns2d_c2r", "ns2d_get_info", "ns2d_sync",
            "ns2d_get_stream", "ns2d_profile_steps", "ns2d_variant_count", "ns2d_variant_info", "ns2d_variant_extra",
-           "ns2d_set_variant", "ns2d_ens_create", "ns2d_ens_member", "ns2d_ens_step", "ns2d_ens_sync",
+           "ns2d_set_variant", "ns2d_ens_create", "ns2d_ens_member", "ns2d_ens_step", "ns2d_ens_autotune", "ns2d_ens_variants", "ns2d_ens_sync",
            "ns2d_ens_destroy", "ns2d_vjp", "ns2d_vjp_device", "ns2d_last_error"]
 
 _lib = None
@@ -122,6 +122,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ns2d_ens_create": ([P(_Params), i, P(vp)], i),
         "ns2d_ens_member": ([vp, i, P(vp)], i),
         "ns2d_ens_step": ([vp, dp, l], i),
+        "ns2d_ens_autotune": ([vp, i], i),
+        "ns2d_ens_variants": ([vp, i, P(i), P(i)], i),
         "ns2d_ens_sync": ([vp], i),
         "ns2d_ens_destroy": ([vp], i),
         "ns2d_vjp": ([vp, dp, d, l, dp, dp, l], i),
@@ -379,6 +381,15 @@ class Ensemble:
     def step(self, dts, nsteps: int) -> None:
         d = np.ascontiguousarray(np.broadcast_to(np.asarray(dts, dtype=np.float64), (len(self.members),)))
         _check(_lib.ns2d_ens_step(self._h, _dptr(d), int(nsteps)))
+
+    def autotune(self, reps: int = 3) -> None:
+        _check(_lib.ns2d_ens_autotune(self._h, int(reps)))
+
+    def variants(self, member: int) -> tuple:
+        """(column-pass, row-pass) shape indices the member's size class uses."""
+        k, r = ctypes.c_int(), ctypes.c_int()
+        _check(_lib.ns2d_ens_variants(self._h, int(member), ctypes.byref(k), ctypes.byref(r)))
+        return k.value, r.value
 
     def sync(self) -> None:
         _check(_lib.ns2d_ens_sync(self._h))

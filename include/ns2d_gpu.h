@@ -184,6 +184,13 @@ int ns2d_ens_create(const ns2d_params* members, int n, ns2d_ens_t** out);
 int ns2d_ens_member(ns2d_ens_t* e, int i, ns2d_t** h);
 /* nsteps RK4CN steps of every member, member i with step dt[i]. */
 int ns2d_ens_step(ns2d_ens_t* e, const double* dt, long nsteps);
+/* The batched passes' autotuner: times every compiled row- and column-pass
+ * shape on all members of each size class (best of reps) and keeps the
+ * fastest; member states are preserved.  ns2d_ens_step runs it once on its
+ * first batched step unless NS2D_ENS_TUNE=0.  ns2d_ens_variants reports the
+ * shapes (ns2d_variant_info indices) a member's class uses. */
+int ns2d_ens_autotune(ns2d_ens_t* e, int reps);
+int ns2d_ens_variants(ns2d_ens_t* e, int member, int* kpass_idx, int* rpass_idx);
 int ns2d_ens_sync(ns2d_ens_t* e);
 int ns2d_ens_destroy(ns2d_ens_t* e);
 

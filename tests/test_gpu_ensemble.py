@@ -1,7 +1,8 @@
 """GPU: the batched ensemble path against independent single-solver runs and
 the oracle.  Members of different sizes, viscosities and step sizes share
-batched launches and a replayed hipGraph; each member must reproduce its own
-plain solve (same kernels, same arithmetic: bit-identical) and the oracle."""
+batched launches (size classes on concurrent streams) and a replayed hipGraph;
+each member must reproduce its own plain solve (bit-identical with the same
+pass shapes, 1e-13 with autotuned ones) and the oracle."""
 import numpy as np
 import pytest
 
@@ -17,8 +18,7 @@ def cfg_of(p):
                        integrator="RK4CN")
 
 
-def test_ensemble_matches_plain_solves_and_oracle():
-    import ns2d
+def _members():
     specs = [(64, 64, 1e-3, 1e-3), (128, 128, 2e-3, 7e-4), (64, 64, 5e-4, 1.5e-3),
              (256, 128, 1e-3, 5e-4), (128, 128, 1e-3, 1e-3), (64, 64, 1e-3, 2e-3)]
     params, w0s, dts = [], [], []
@@ -27,8 +27,43 @@ def test_ensemble_matches_plain_solves_and_oracle():
         params.append(p)
         w0s.append(O.ic_random_spectrum(p, seed=100 + i, k0=4.0 + i))
         dts.append(dt)
+    return params, w0s, dts
+
+
+def test_autotuned_ensemble_matches_plain_solves_and_oracle():
+    """The autotuner may pick other pass shapes than a plain solver's
+    defaults (different FFT radices, so different roundoff): members then
+    agree with their plain solves to 1e-13 and with the oracle to 1e-12."""
+    import ns2d
+    params, w0s, dts = _members()
+    nsteps = 9
+    with ns2d.Ensemble([cfg_of(p) for p in params]) as ens:
+        for i, w in enumerate(w0s):
+            ens[i].set_state(w)
+        ens.step(dts, nsteps)
+        got = [ens[i].get_state() for i in range(len(ens))]
+        shapes = [ens.variants(i) for i in range(len(ens))]
+    for i, p in enumerate(params):
+        k, r = shapes[i]
+        assert 0 <= k < len(ns2d.kpass_variants(p.Nx)) and 0 <= r < len(ns2d.rpass_variants(p.Ny))
+        with ns2d.Solver(cfg_of(p)) as s:
+            s.set_state(w0s[i])
+            s.spectral_step(dts[i], nsteps)
+            plain = s.get_state()
+        assert np.linalg.norm(got[i] - plain) / np.linalg.norm(plain) < 1e-13, i
+        st = O.State(w0s[i], p)
+        O.step(st, dts[i], nsteps)
+        assert np.linalg.norm(got[i] - st.w_hat) / np.linalg.norm(st.w_hat) < 1e-12
+
+
+def test_ensemble_matches_plain_solves_and_oracle(monkeypatch):
+    """Without the autotuner the batched launches use the plain solver's
+    shapes: each member is bit-identical to its own plain solve."""
+    import ns2d
+    monkeypatch.setenv("NS2D_ENS_TUNE", "0")
+    params, w0s, dts = _members()
     nsteps = 7    # odd: graph periods plus a single batched step
-    print("ensemble of", len(specs), flush=True)
+    print("ensemble of", len(params), flush=True)
     with ns2d.Ensemble([cfg_of(p) for p in params]) as ens:
         for i, w in enumerate(w0s):
             ens[i].set_state(w)
