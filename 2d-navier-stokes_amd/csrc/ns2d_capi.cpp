@@ -80,8 +80,11 @@ struct ns2d_handle {
     const double* lam;
   } gkey{};
   // adjoint buffers (allocated on first use)
-  double2* tape = nullptr;     // [slot][4 fields][M], slot = 4*step + stage
+  double2* tape = nullptr;     // the tape in use: [slot][4 fields][M], slot = 4*step + stage
+  double2* tape_buf[2] = {nullptr, nullptr};  // second one only when it fits (overlap)
   long tape_slots = 0;
+  hipStream_t stream2 = nullptr;  // recompute stream of the adjoint
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double2* ckpt = nullptr;     // [segment][M]
   long ckpt_n = 0;
   double2 *lamp[2] = {nullptr, nullptr}, *lacc = nullptr, *lbuf = nullptr, *bbuf[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -561,7 +564,8 @@ int ns2d_destroy(ns2d_t* h) {
     if (b) (void)hipFree(b);
   for (double* t : h->lam_tab)
     if (t) (void)hipFree(t);
-  double2* adj[] = {h->tape, h->ckpt, h->lamp[0], h->lamp[1], h->lacc, h->lbuf, h->bbuf[0], h->bbuf[1], h->bbuf[2],
+  if (h->stream2) (void)hipStreamSynchronize(h->stream2);
+  double2* adj[] = {h->tape_buf[0], h->tape_buf[1], h->ckpt, h->lamp[0], h->lamp[1], h->lacc, h->lbuf, h->bbuf[0], h->bbuf[1], h->bbuf[2],
                     h->bbuf[3]};
   for (double2* b : adj)
     if (b) (void)hipFree(b);
@@ -573,6 +577,9 @@ int ns2d_destroy(ns2d_t* h) {
   drop_graph(h);
   for (auto& g : h->adj_graph)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+  if (h->stream2) (void)hipStreamDestroy(h->stream2);
   if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return NS2D_OK;
@@ -1340,13 +1347,33 @@ namespace {
 int adj_alloc(ns2d_handle* h, long slots, long nckpt) {
   const size_t fb = h->M * sizeof(double2);
   if (h->tape_slots < slots) {
-    if (h->tape) (void)hipFree(h->tape);
-    h->tape = nullptr;
+    for (double2*& t : h->tape_buf) {
+      if (t) (void)hipFree(t);
+      t = nullptr;
+    }
     h->tape_slots = 0;
-    if (hipMalloc((void**)&h->tape, fb * 4 * (size_t)slots) != hipSuccess)
-      return fail(NS2D_ERR_HIP, "adjoint tape of %ld stage slots (%.1f GB) does not fit", slots,
-                  fb * 4.0 * slots / 1e9);
+    const size_t tb = fb * 4 * (size_t)slots;
+    if (hipMalloc((void**)&h->tape_buf[0], tb) != hipSuccess)
+      return fail(NS2D_ERR_HIP, "adjoint tape of %ld stage slots (%.1f GB) does not fit", slots, tb / 1e9);
+    // a second tape lets the recompute of one segment overlap the adjoint
+    // sweep of the next; taken only while it leaves half the free memory
+    // (NS2D_ADJ_OVERLAP=0 keeps one tape and the sequential order)
+    const char* ov = std::getenv("NS2D_ADJ_OVERLAP");
+    size_t fr = 0, tot = 0;
+    if (!(ov && ov[0] == '0') && hipMemGetInfo(&fr, &tot) == hipSuccess && tb <= fr / 2) {
+      if (hipMalloc((void**)&h->tape_buf[1], tb) != hipSuccess) {
+        (void)hipGetLastError();
+        h->tape_buf[1] = nullptr;
+      }
+    }
     h->tape_slots = slots;
+  }
+  h->tape = h->tape_buf[0];
+  if (h->tape_buf[1] && !h->stream2) {
+    if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess)
+      return fail(NS2D_ERR_HIP, "adjoint stream create failed");
   }
   if (h->ckpt_n < nckpt) {
     if (h->ckpt) (void)hipFree(h->ckpt);
@@ -1429,6 +1456,7 @@ int vjp_impl(ns2d_handle* h, double dt, long n, long ckpt_every, double2** res) 
   for (long seg = 0; seg < S; ++seg) {
     HIPCHK(hipMemcpyAsync(h->ckpt + (size_t)seg * h->M, h->w[h->wcur], fb, hipMemcpyDeviceToDevice, h->stream));
     const long cnt = (seg == S - 1) ? n - seg * C : C;
+    if (seg == S - 1) h->tape = (h->tape_buf[1] && S > 1) ? h->tape_buf[seg & 1] : h->tape_buf[0];
     rc = (seg == S - 1) ? taped_forward(h, dt, cnt) : step_impl(h, dt, cnt);
     if (rc) {
       h->p.check_every = check0;
@@ -1437,19 +1465,32 @@ int vjp_impl(ns2d_handle* h, double dt, long n, long ckpt_every, double2** res) 
     h->inv_valid = false;
   }
   h->p.check_every = check0;
-  // backward sweep
+  // backward sweep.  A segment's stages are recomputed onto a tape, then
+  // swept backwards.  With two tapes the recompute of segment seg-1 (second
+  // stream, state buffers) runs while the adjoint sweep of segment seg (own
+  // stream, adjoint buffers) runs: they share no buffer but the tapes.
   int lc = 0;   // lamp[lc] holds dJ/dw at the current step boundary
   bool started = false;
-  // one segment: recompute its stages onto the tape (all but the last
-  // segment, taped above), then the adjoint stages backwards over its steps
-  auto segment = [&](long seg, long cnt) -> int {
-    if (seg != S - 1) {
-      h->iters = seg * C;
-      h->hyper_eff = h->p.hyper_flag;
-      if (h->iters > 0) after_rk_update(h);
-      int r = taped_forward(h, dt, cnt);
-      if (r) return r;
-    }
+  const bool dbl = h->tape_buf[1] != nullptr && S > 1;
+  auto tape_of = [&](long seg) { return dbl ? h->tape_buf[seg & 1] : h->tape_buf[0]; };
+  // segment seg's stages onto its tape, from its checkpoint already copied
+  // into the state buffer (that copy stays outside the captured graphs: its
+  // source address is the one thing that changes between segments)
+  auto recompute = [&](long seg, hipStream_t st) -> int {
+    const long cnt = (seg == S - 1) ? n - seg * C : C;
+    hipStream_t keep = h->stream;
+    h->stream = st;
+    h->iters = seg * C;
+    h->hyper_eff = h->p.hyper_flag;
+    if (h->iters > 0) after_rk_update(h);
+    h->tape = tape_of(seg);
+    const int r = taped_forward(h, dt, cnt);
+    h->stream = keep;
+    return r;
+  };
+  auto adjoint = [&](long seg) -> int {
+    const long cnt = (seg == S - 1) ? n - seg * C : C;
+    h->tape = tape_of(seg);
     for (long j = cnt - 1; j >= 0; --j) {
       const long ng = seg * C + j;
       KAdjArgs ka{};
@@ -1499,21 +1540,47 @@ int vjp_impl(ns2d_handle* h, double dt, long n, long ckpt_every, double2** res) 
     }
     return NS2D_OK;
   };
+  // one backward iteration: the adjoint of seg and, ahead of it, segment
+  // seg-1 onto its tape (overlapped when two tapes exist)
+  auto iteration = [&](long seg) -> int {
+    if (!dbl) {
+      int r = seg != S - 1 ? recompute(seg, h->stream) : NS2D_OK;
+      return r ? r : adjoint(seg);
+    }
+    // seg's tape is ready: the forward sweep taped S-1, each iteration the one before
+    if (seg >= 1) {
+      HIPCHK(hipEventRecord(h->ev_fork, h->stream));
+      HIPCHK(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+      int r = recompute(seg - 1, h->stream2);
+      if (r) return r;
+      r = adjoint(seg);
+      if (r) return r;
+      HIPCHK(hipEventRecord(h->ev_join, h->stream2));
+      HIPCHK(hipStreamWaitEvent(h->stream, h->ev_join, 0));
+      return NS2D_OK;
+    }
+    return adjoint(seg);
+  };
   for (long seg = S - 1; seg >= 0; --seg) {
     const long cnt = (seg == S - 1) ? n - seg * C : C;
-    if (seg != S - 1)
-      HIPCHK(hipMemcpyAsync(h->w[h->wcur], h->ckpt + (size_t)seg * h->M, fb, hipMemcpyDeviceToDevice, h->stream));
-    // full middle segments repeat the same launches: one capture per
-    // (state buffer, adjoint buffer) parity, replayed for the others
-    const bool graphable = h->use_graph && !h->profiling && started && seg != S - 1 && seg != 0 && C >= 2;
+    // the checkpoint this iteration recomputes from; with two tapes the
+    // copy waits for nothing but the previous iteration's join
+    const long rseg = dbl ? seg - 1 : (seg != S - 1 ? seg : -1);
+    if (rseg >= 0)
+      HIPCHK(hipMemcpyAsync(h->w[h->wcur], h->ckpt + (size_t)rseg * h->M, fb, hipMemcpyDeviceToDevice, h->stream));
+    // middle iterations repeat the same launches: one capture per parity of
+    // (tape, state buffer, adjoint buffer), replayed for the others
+    const bool graphable = h->use_graph && !h->profiling && started && seg != S - 1 && C >= 2 &&
+                           seg >= (dbl ? 2 : 1);
     if (!graphable) {
-      rc = segment(seg, cnt);
+      rc = iteration(seg);
       if (rc) return rc;
       continue;
     }
+    const double2* tkey = tape_of(seg);
     ns2d_handle::AdjGraph* g = nullptr;
     for (auto& c : h->adj_graph)
-      if (c.exec && c.dt == dt && c.C == C && c.wcur == h->wcur && c.lc == lc && c.lam == lam1 && c.tape == h->tape)
+      if (c.exec && c.dt == dt && c.C == C && c.wcur == h->wcur && c.lc == lc && c.lam == lam1 && c.tape == tkey)
         g = &c;
     if (!g) {
       g = !h->adj_graph[0].exec ? &h->adj_graph[0] : !h->adj_graph[1].exec ? &h->adj_graph[1] : &h->adj_graph[seg & 1];
@@ -1521,7 +1588,7 @@ int vjp_impl(ns2d_handle* h, double dt, long n, long ckpt_every, double2** res) 
       g->exec = nullptr;
       const int wcur0 = h->wcur, lc0 = lc;
       HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      rc = segment(seg, cnt);
+      rc = iteration(seg);
       hipGraph_t gr = nullptr;
       const hipError_t ec = hipStreamEndCapture(h->stream, &gr);
       if (rc) {
@@ -1537,13 +1604,16 @@ int vjp_impl(ns2d_handle* h, double dt, long n, long ckpt_every, double2** res) 
       g->wcur = wcur0;
       g->lc = lc0;
       g->lam = lam1;
-      g->tape = h->tape;
+      g->tape = tkey;
       // the capture advanced the host-side parities; the replay does the work
       h->wcur = wcur0;
       lc = lc0;
     }
     HIPCHK(hipGraphLaunch(g->exec, h->stream));
-    h->wcur ^= (int)(cnt & 1);
+    // host-side parities as the iteration leaves them: the recompute (of seg-1
+    // with two tapes, of seg otherwise) flips the state buffer once per step,
+    // the adjoint of seg flips lc once per step
+    h->wcur ^= (int)(C & 1);
     lc ^= (int)(cnt & 1);
   }
   *res = h->lamp[lc];

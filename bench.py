@@ -164,7 +164,7 @@ def adjoint_leg(n=512, steps=2000, ckpt=100, seed=5):
         c[0, 0] = c[0, -1] = 1.0
         lam = 2.0 * c * wT * (4 * np.pi ** 2) * (0.5 / float(n * n) ** 2)
         lam[0, 0] = 0.0
-        s.vjp(w0, dt, 20, lam, ckpt_every=10)   # warm-up (allocations)
+        s.vjp(w0, dt, 4 * ckpt, lam, ckpt_every=ckpt)   # warm-up (tapes, streams, graphs)
         t0 = time.perf_counter()
         g = s.vjp(w0, dt, steps, lam, ckpt_every=ckpt)
         t_adj = time.perf_counter() - t0

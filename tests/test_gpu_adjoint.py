@@ -138,6 +138,30 @@ def test_vjp_graph_replayed_segments(monkeypatch):
     assert np.linalg.norm(g3 - g_ref) / np.linalg.norm(g_ref) < 1e-11
 
 
+@pytest.mark.parametrize("ckpt", [4, 3, 7])
+def test_vjp_overlapped_recompute(monkeypatch, ckpt):
+    """Two tapes (the recompute of segment s-1 on a second stream beside the
+    adjoint sweep of segment s) against one tape in sequential order, with and
+    without graph replay: bit for bit, and against torch autograd."""
+    p = O.Params(64, 64, nu=1e-3, hyper_flag=0, hyper_pow=1.0, quirks=0)
+    w0 = O.ic_random_spectrum(p, seed=13, k0=5.0)
+    dt, n = 2e-3, 29
+    J, g_ref, wT = torch_grad(p, w0, dt, n)
+    lam = enstrophy_grad(p, wT)
+    got = {}
+    for ov in ("1", "0"):
+        for gr in ("1", "0"):
+            monkeypatch.setenv("NS2D_ADJ_OVERLAP", ov)
+            monkeypatch.setenv("NS2D_GRAPH", gr)
+            with make(p) as s:
+                got[ov, gr] = s.vjp(w0, dt, n, lam, ckpt_every=ckpt)
+                # a second call reuses tapes, streams and cached graphs
+                np.testing.assert_array_equal(s.vjp(w0, dt, n, lam, ckpt_every=ckpt), got[ov, gr])
+    for k, g in got.items():
+        np.testing.assert_array_equal(g, got["0", "0"], err_msg=str(k))
+    assert np.linalg.norm(got["1", "1"] - g_ref) / np.linalg.norm(g_ref) < 1e-11
+
+
 def test_vjp_rejects_reference_conjugacy_quirk():
     import ns2d
     p = O.Params(32, 32, nu=1e-3)
