@@ -61,6 +61,7 @@ struct ns2d_handle {
   double* lam_tab[4] = {nullptr, nullptr, nullptr, nullptr};  // per VISC_* kind
   double* filt_tab = nullptr;
   double2 *conj[2] = {nullptr, nullptr}, *tw = nullptr;
+  double2* pairbuf = nullptr;  // column pass: held columns of the packed (0, Ny/2) pair, [split][2][Nx]
   unsigned long long* stat = nullptr;
   double* meas = nullptr;
   double* wreal = nullptr;
@@ -169,6 +170,7 @@ KPassArgs kbase(ns2d_handle* h) {
   a.kout = h->kout;
   a.stat = h->stat;
   a.tw = h->tw;
+  a.pairbuf = h->pairbuf;
   a.nyf = h->nyf;
   a.ld = h->ld;
   a.ph = h->ph;
@@ -379,10 +381,9 @@ int ns2d_variant_info(int pass, int n, int idx, int* shape4) {
 int ns2d_variant_extra(int pass, int n, int idx, int* extra4) {
   if (!extra4) return fail(NS2D_ERR_INVALID, "null argument");
   int rc = -1;
-  if (pass == NS2D_PASS_COLUMN) rc = kpass_variant_extra(n, idx, &extra4[0], &extra4[1], &extra4[2]);
-  if (pass == NS2D_PASS_ROW) rc = rpass_variant_extra(n, idx, &extra4[0], &extra4[1], &extra4[2]);
+  if (pass == NS2D_PASS_COLUMN) rc = kpass_variant_extra(n, idx, &extra4[0], &extra4[1], &extra4[2], &extra4[3]);
+  if (pass == NS2D_PASS_ROW) rc = rpass_variant_extra(n, idx, &extra4[0], &extra4[1], &extra4[2], &extra4[3]);
   if (rc) return fail(NS2D_ERR_INVALID, "no variant %d of pass %d for n=%d", idx, pass, n);
-  extra4[3] = 0;
   return NS2D_OK;
 }
 
@@ -500,6 +501,8 @@ static int create_impl(const ns2d_params* p, hipStream_t ext_stream, ns2d_t** ou
       return cleanup(fail(NS2D_ERR_HIP, "hipMalloc failed"));
     (void)hipMemsetAsync(h->conj[i], 0, sizeof(double2) * p->nx, h->stream);
   }
+  if (hipMalloc((void**)&h->pairbuf, sizeof(double2) * 8 * p->nx) != hipSuccess)
+    return cleanup(fail(NS2D_ERR_HIP, "hipMalloc failed"));
   if (hipMalloc((void**)&h->stat, sizeof(unsigned long long) * STAT_N) != hipSuccess)
     return cleanup(fail(NS2D_ERR_HIP, "hipMalloc failed"));
   if (hipMalloc((void**)&h->meas, sizeof(double) * NQ * h->nspect) != hipSuccess)
@@ -559,7 +562,7 @@ int ns2d_destroy(ns2d_t* h) {
   (void)hipSetDevice(h->p.device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   double2* bufs[] = {h->w[0], h->w[1], h->acc, h->g[0], h->g[1], h->g[2], h->g[3], h->nbuf, h->kout, h->sbuf,
-                     h->hist[0], h->hist[1], h->hist[2], h->hist[3], h->conj[0], h->conj[1], h->tw};
+                     h->hist[0], h->hist[1], h->hist[2], h->hist[3], h->conj[0], h->conj[1], h->tw, h->pairbuf};
   for (double2* b : bufs)
     if (b) (void)hipFree(b);
   for (double* t : h->lam_tab)

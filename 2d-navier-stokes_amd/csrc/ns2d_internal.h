@@ -66,6 +66,7 @@ struct KPassArgs {
   double2* conj_dst;
   unsigned long long* stat;
   const double2* tw;
+  double2* pairbuf;      // packed (0, Ny/2) column pair: held columns, [split][2][Nx]
   int nyf, ld;
   const double* lam;     // per-mode D_fac/dt of the CN update, [Nx][ld]
   int mode, epi;
@@ -138,8 +139,8 @@ int kpass_variant_info(int nx, int idx, int* C, int* E, int* T, int* KS);
 hipError_t launch_rpass(int nx, int ny, int variant, const RPassArgs& a, hipStream_t s);
 int rpass_variant_count(int ny);
 int rpass_variant_info(int ny, int idx, int* TLR, int* E, int* T, int* PAIRS);
-int rpass_variant_extra(int ny, int idx, int* W, int* PS, int* OFF);
-int kpass_variant_extra(int nx, int idx, int* W, int* PS, int* OFF);
+int rpass_variant_extra(int ny, int idx, int* W, int* PS, int* OFF, int* PK);
+int kpass_variant_extra(int nx, int idx, int* W, int* PS, int* OFF, int* PK);
 hipError_t launch_r2c_rows(int nx, int ny, const double* in, double2* out, const double2* tw, int ld,
                            hipStream_t s);
 hipError_t launch_measures(int nbins, const MeasArgs& a, hipStream_t s);

@@ -71,19 +71,44 @@ __device__ __forceinline__ double dealias_factor(const PhysArgs& ph, int kx, int
 // ---------------------------------------------------------------------------
 // kpass: column pass in k-space
 //
-// A work item is a group of C adjacent ky-columns.  Thread tid owns column
+// A work item is a group of C adjacent column slots.  Thread tid owns slot
 // c = tid % C and the rows x = l + TL*e (l = tid / C, TL = T/C, e < E = L/TL),
 // so each global access of a wave covers C contiguous complex values of a row,
-// and the column's x-transform runs in registers (fft_reg) with the column's
-// own LDS exchange buffer.  KS workgroups share a column group: each redoes the
+// and the column's x-transform runs in registers (fft_reg) with the slot's own
+// LDS exchange buffer.  KS workgroups share a column group: each redoes the
 // forward transform and takes 4/KS of the inverse fields; only split 0 writes
 // the integrator state (the new state goes to a separate buffer w_out, so no
 // workgroup reads what another writes).
+//
+// PK (packed pair): slot s >= 1 is column ky = s and slot 0 carries both the
+// ky = 0 and the ky = Ny/2 = H column, so Ny/2 slots cover the Ny/2+1 columns
+// (an even count: no work item is left holding the odd column alone).  Two
+// columns go through one complex transform in both directions:
+//   forward  the row pass leaves N(x,0) and N(x,H) real (they are the
+//            y-transforms' k = 0 and k = H terms of real rows), so
+//            Z = X[N(.,0) + i N(.,H)] gives K0 = (Z + conj Z(-kx))/2 and
+//            KH = (Z - conj Z(-kx))/(2i);
+//   inverse  the row pass uses one real number per entry of these columns:
+//            after its i ky factor it keeps the real part (FFTW's c2r
+//            projection), i.e. Re G2, Re G3, Re G4 at ky = 0 and H, Im G1 and
+//            Im G4 at H (-H Im G = Re(i H G)), nothing of G1 at ky = 0.  With
+//            Re X^-1[P] = X^-1[(P + conj P(-kx))/2], one transform of
+//            (P + conj P(-kx))/2 + i (Q + conj Q(-kx))/2 gives Re X^-1[P] and
+//            Re X^-1[Q]; the four transforms are
+//              G1 slot: P = -i psi_H, Q = -i S_H  -> Im G1(x,H), Im G4(x,H)
+//              G2 slot: P = -i kx psi_0, Q = -i kx psi_H  -> Re G2(x,0), Re G2(x,H)
+//              G3 slot: P = i kx S_0, Q = i kx S_H        -> Re G3(x,0), Re G3(x,H)
+//              G4 slot: P = S_0, Q = S_H                  -> Re G4(x,0), Re G4(x,H)
+//            (G4(x,H) is written as two 8-byte halves from two slots; the
+//            unused parts are written as zeros.)
+// The columns the transforms need wait in pairbuf ([split][2][Nx], per
+// workgroup split): KH between the forward transform and its epilogue, then
+// the next stage inputs S_0 and S_H.
 // ---------------------------------------------------------------------------
 // PS: LDS padding (one slot per 2^PS elements); OFF: extra 16-B slots between
-// the columns' lines, which sets where the neighbouring column's accesses of
-// one wave fall among the LDS banks (adjacent lanes work on adjacent columns)
-template <int L, int C, int T, int KS, int PS, int OFF>
+// the slots' lines, which sets where the neighbouring slot's accesses of one
+// wave fall among the LDS banks (adjacent lanes work on adjacent slots)
+template <int L, int C, int T, int KS, int PS, int OFF, int PK>
 __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
   constexpr int TL = T / C;        // threads per column
   constexpr int E = L / TL;        // rows per thread
@@ -95,15 +120,19 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
 
   const int tid = threadIdx.x;
   const int g = wi / KS, split = wi % KS;
-  const int ky0 = g * C;
   const int c = tid % C, l = tid / C;
-  const int ky = ky0 + c;
+  const int ky = g * C + c;        // the slot; with PK slot 0 is the (0, Ny/2) pair
   const int nyf = a.nyf, ld = a.ld, nx = L;
-  const bool colv = ky < nyf;
+  const int H = nyf - 1;           // ky of the Nyquist column
+  const bool colv = ky < (PK ? H : nyf);
+  const bool pair = PK && ky == 0;
+  const bool pblk = PK && g == 0;  // uniform per workgroup
   const bool writer = split == 0;
   const double2* __restrict__ tw = a.tw;
   const PhysArgs& ph = a.ph;
   double2* buf = lds + c * LS;
+  // the pair's held columns (see above)
+  double2* __restrict__ hold = a.pairbuf + (size_t)split * 2 * L;
 
   double2 R[E];  // forward result K, then the next stage input S
 
@@ -112,138 +141,178 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int x = l + TL * e;
-      R[e] = colv ? a.nin[(size_t)x * ld + ky] : make_double2(0.0, 0.0);
+      if (pair) R[e] = make_double2(a.nin[(size_t)x * ld].x, a.nin[(size_t)x * ld + H].x);
+      else R[e] = colv ? a.nin[(size_t)x * ld + ky] : make_double2(0.0, 0.0);
       if ((e & 3) == 3) asm volatile("" ::: "memory");
     }
     fft_reg<L, TL, -1, BLOCK, PS>(R, buf, tw, opaque(l));
-    const double nrm = ph.norm2;
+    if (pblk) {
+      if (pair) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int kx = kx_of_row(l + TL * e, nx);
-      double2 v = R[e];
-      if (a.mode & KP_RAW) {
-        // plain unnormalised forward transform (initial conditions)
-      } else if (kx == 0 && ky == 0) {
-        v = make_double2(0.0, 0.0);
-      } else {
-        v = cscale(nrm, v);                                  // solver.c:1005
-        v = cscale(dealias_factor(ph, kx, ky), v);           // solver.c:1019
-        if (ph.force_on && kx == 0 && ky == ph.force_k)      // solver.c:1022-1027
-          v.x += ph.force_val;
+        for (int e = 0; e < E; ++e) buf[padx_s<PS>(l + TL * e)] = R[e];
       }
-      R[e] = v;
+      xsync<BLOCK>();
+      if (pair) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int x = l + TL * e;
+          const double2 z = R[e], zm = buf[padx_s<PS>((L - x) & (L - 1))];
+          R[e] = make_double2(0.5 * (z.x + zm.x), 0.5 * (z.y - zm.y));    // K0
+          hold[L + x] = make_double2(0.5 * (z.y + zm.y), -0.5 * (z.x - zm.x));  // KH
+        }
+      }
+      xsync<BLOCK>();
     }
   }
 
-  // ---------------- EPI: integrator stage ----------------------------------
+  // ---------------- EPI: integrator stage, per column of the slot ----------
   // The state w as the epilogue sees it, with the ForceConjugacy write of the
   // RHS just finished: w_hat[kx<0, 0] = conj(stage_input[|kx|, 0])
   // (solver.c:2096-2102).  Padding columns (ky >= Ny/2+1) carry zeros.
-  const bool conj_fix = (a.mode & KP_CONJFIX) && ky0 == 0 && c == 0;
-  auto state_at = [&](int it) -> double2 {
-    const int i = l + TL * it;
-    const int kx = kx_of_row(i, nx);
-    if (conj_fix && kx < 0) return cconj(a.conj_src[-kx]);
-    return a.w_in[(size_t)i * ld + ky];
-  };
-  auto index_of = [&](int it) -> size_t { return (size_t)(l + TL * it) * ld + ky; };
-  switch (a.epi) {
-    case EPI_FROM_STATE:
+#pragma unroll 1
+  for (int q = 0; q < (pair ? 2 : 1); ++q) {
+    // opaque: keeps the row offsets from being hoisted out of the loop and
+    // held live across it
+    const int kq = opaque(q ? H : ky);
+    const int lq = opaque(l);
+    auto index_of = [&](int it) -> size_t { return (size_t)(lq + TL * it) * ld + kq; };
+    if (a.mode & KP_FWD) {
+      if (q == 1) {
 #pragma unroll
-      for (int it = 0; it < E; ++it) {
-        R[it] = colv ? a.w_in[index_of(it)] : make_double2(0.0, 0.0);
-        if ((it & 3) == 3) asm volatile("" ::: "memory");
+        for (int e = 0; e < E; ++e) R[e] = hold[L + lq + TL * e];
       }
-      break;
-    case EPI_FROM_INPUT:
+      const double nrm = ph.norm2;
 #pragma unroll
-      for (int it = 0; it < E; ++it) {
-        R[it] = colv ? a.sin[index_of(it)] : make_double2(0.0, 0.0);
-        if ((it & 3) == 3) asm volatile("" ::: "memory");
-      }
-      break;
-    case EPI_RHS:
-      if (writer && colv) {
-#pragma unroll
-        for (int it = 0; it < E; ++it) {
-          a.kout[index_of(it)] = R[it];
-          const int kx = kx_of_row(l + TL * it, nx);
-          if (conj_fix && kx < 0) a.w_out[index_of(it)] = cconj(a.conj_src[-kx]);
+      for (int e = 0; e < E; ++e) {
+        const int kx = kx_of_row(lq + TL * e, nx);
+        double2 v = R[e];
+        if (a.mode & KP_RAW) {
+          // plain unnormalised forward transform (initial conditions)
+        } else if (kx == 0 && kq == 0) {
+          v = make_double2(0.0, 0.0);
+        } else {
+          v = cscale(nrm, v);                                  // solver.c:1005
+          v = cscale(dealias_factor(ph, kx, kq), v);           // solver.c:1019
+          if (ph.force_on && kx == 0 && kq == ph.force_k)      // solver.c:1022-1027
+            v.x += ph.force_val;
         }
+        R[e] = v;
       }
-      break;
-    case EPI_RK1:
-      if (colv) {
+    }
+    const bool conj_fix = (a.mode & KP_CONJFIX) && kq == 0;
+    auto state_at = [&](int it) -> double2 {
+      const int i = lq + TL * it;
+      const int kx = kx_of_row(i, nx);
+      if (conj_fix && kx < 0) return cconj(a.conj_src[-kx]);
+      return a.w_in[(size_t)i * ld + kq];
+    };
+    switch (a.epi) {
+      case EPI_FROM_STATE:
 #pragma unroll
         for (int it = 0; it < E; ++it) {
-          const double2 k = R[it];
-          if (writer) {
-            a.acc[index_of(it)] = cscale(RK4_B1, k);
-            if (a.hist_store) a.hist_store[index_of(it)] = k;
-          }
-          R[it] = cadd(state_at(it), cscale(a.coef, k));     // solver.c:690
+          R[it] = colv ? a.w_in[index_of(it)] : make_double2(0.0, 0.0);
           if ((it & 3) == 3) asm volatile("" ::: "memory");
         }
-      }
-      break;
-    case EPI_RK2:
-    case EPI_RK3:
-      if (colv) {
-        const double b = (a.epi == EPI_RK2) ? RK4_B2 : RK4_B3;
+        break;
+      case EPI_FROM_INPUT:
 #pragma unroll
         for (int it = 0; it < E; ++it) {
-          const double2 k = R[it];
-          if (writer) a.acc[index_of(it)] = cadd(a.acc[index_of(it)], cscale(b, k));
-          R[it] = cadd(state_at(it), cscale(a.coef, k));     // solver.c:729,768
+          R[it] = colv ? a.sin[index_of(it)] : make_double2(0.0, 0.0);
           if ((it & 3) == 3) asm volatile("" ::: "memory");
         }
-      }
-      break;
-    case EPI_RK4CN:
-      if (colv) {
+        break;
+      case EPI_RHS:
+        if (writer && colv) {
 #pragma unroll
-        for (int it = 0; it < E; ++it) {
-          const int kx = kx_of_row(l + TL * it, nx);
-          double2 nw = make_double2(0.0, 0.0);               // (0,0) mode: solver.c:860
-          if (kx != 0 || ky != 0) {
-            const double2 sum = cadd(a.acc[index_of(it)], cscale(RK4_B4, R[it]));
-            const double D = a.dt * a.lam[index_of(it)];
-            nw = cadd(cscale((2.0 - D) / (2.0 + D), state_at(it)), cscale(2.0 * a.dt / (2.0 + D), sum));  // :853
+          for (int it = 0; it < E; ++it) {
+            a.kout[index_of(it)] = R[it];
+            const int kx = kx_of_row(lq + TL * it, nx);
+            if (conj_fix && kx < 0) a.w_out[index_of(it)] = cconj(a.conj_src[-kx]);
           }
-          if (writer) a.w_out[index_of(it)] = nw;
-          R[it] = nw;
-          if ((it & 3) == 3) asm volatile("" ::: "memory");
         }
-      }
-      break;
-    case EPI_AB4:
-      if (colv) {
+        break;
+      case EPI_RK1:
+        if (colv) {
 #pragma unroll
-        for (int it = 0; it < E; ++it) {
-          const int kx = kx_of_row(l + TL * it, nx);
-          const size_t idx = index_of(it);
-          const double2 k = R[it];
-          double2 nw = make_double2(0.0, 0.0);               // (0,0) mode: solver.c:337
-          if (kx != 0 || ky != 0) {
-            const double2 sum = cadd(cadd(cadd(cscale(AB4_1, k), cscale(AB4_2, a.h_new[idx])),
-                                          cscale(AB4_3, a.h_mid[idx])), cscale(AB4_4, a.h_old[idx]));
-            const double D = a.dt * a.lam[idx];
-            nw = cadd(cscale((2.0 - D) / (2.0 + D), state_at(it)), cscale(2.0 * a.dt / (2.0 + D), sum));  // :333
+          for (int it = 0; it < E; ++it) {
+            const double2 k = R[it];
+            if (writer) {
+              a.acc[index_of(it)] = cscale(RK4_B1, k);
+              if (a.hist_store) a.hist_store[index_of(it)] = k;
+            }
+            R[it] = cadd(state_at(it), cscale(a.coef, k));     // solver.c:690
+            if ((it & 3) == 3) asm volatile("" ::: "memory");
           }
-          if (writer) {
-            a.h_store[idx] = k;   // newest history entry (solver.c:346-348)
-            a.w_out[idx] = nw;
-          }
-          R[it] = nw;
-          if ((it & 3) == 3) asm volatile("" ::: "memory");
         }
-      }
-      break;
-    default: break;
-  }
-  if (!colv) {
+        break;
+      case EPI_RK2:
+      case EPI_RK3:
+        if (colv) {
+          const double b = (a.epi == EPI_RK2) ? RK4_B2 : RK4_B3;
 #pragma unroll
-    for (int it = 0; it < E; ++it) R[it] = make_double2(0.0, 0.0);
+          for (int it = 0; it < E; ++it) {
+            const double2 k = R[it];
+            if (writer) a.acc[index_of(it)] = cadd(a.acc[index_of(it)], cscale(b, k));
+            R[it] = cadd(state_at(it), cscale(a.coef, k));     // solver.c:729,768
+            if ((it & 3) == 3) asm volatile("" ::: "memory");
+          }
+        }
+        break;
+      case EPI_RK4CN:
+        if (colv) {
+#pragma unroll
+          for (int it = 0; it < E; ++it) {
+            const int kx = kx_of_row(lq + TL * it, nx);
+            double2 nw = make_double2(0.0, 0.0);               // (0,0) mode: solver.c:860
+            if (kx != 0 || kq != 0) {
+              const double2 sum = cadd(a.acc[index_of(it)], cscale(RK4_B4, R[it]));
+              const double D = a.dt * a.lam[index_of(it)];
+              nw = cadd(cscale((2.0 - D) / (2.0 + D), state_at(it)), cscale(2.0 * a.dt / (2.0 + D), sum));  // :853
+            }
+            if (writer) a.w_out[index_of(it)] = nw;
+            R[it] = nw;
+            if ((it & 3) == 3) asm volatile("" ::: "memory");
+          }
+        }
+        break;
+      case EPI_AB4:
+        if (colv) {
+#pragma unroll
+          for (int it = 0; it < E; ++it) {
+            const int kx = kx_of_row(lq + TL * it, nx);
+            const size_t idx = index_of(it);
+            const double2 k = R[it];
+            double2 nw = make_double2(0.0, 0.0);               // (0,0) mode: solver.c:337
+            if (kx != 0 || kq != 0) {
+              const double2 sum = cadd(cadd(cadd(cscale(AB4_1, k), cscale(AB4_2, a.h_new[idx])),
+                                            cscale(AB4_3, a.h_mid[idx])), cscale(AB4_4, a.h_old[idx]));
+              const double D = a.dt * a.lam[idx];
+              nw = cadd(cscale((2.0 - D) / (2.0 + D), state_at(it)), cscale(2.0 * a.dt / (2.0 + D), sum));  // :333
+            }
+            if (writer) {
+              a.h_store[idx] = k;   // newest history entry (solver.c:346-348)
+              a.w_out[idx] = nw;
+            }
+            R[it] = nw;
+            if ((it & 3) == 3) asm volatile("" ::: "memory");
+          }
+        }
+        break;
+      default: break;
+    }
+    if (!colv) {
+#pragma unroll
+      for (int it = 0; it < E; ++it) R[it] = make_double2(0.0, 0.0);
+    }
+    // ky=0 column of the next stage input, for the next ForceConjugacy
+    if ((a.mode & KP_INV) && writer && kq == 0 && a.conj_dst) {
+#pragma unroll
+      for (int it = 0; it < E; ++it) a.conj_dst[lq + TL * it] = R[it];
+    }
+    if (pair && (a.mode & KP_INV)) {
+#pragma unroll
+      for (int it = 0; it < E; ++it) hold[q * L + lq + TL * it] = R[it];
+    }
   }
 
   // blow-up bookkeeping for the reference's per-step SystemCheck
@@ -262,51 +331,96 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
 
   if (!(a.mode & KP_INV)) return;
 
-  // ky=0 column of the next stage input, for the next ForceConjugacy
-  if (writer && ky0 == 0 && c == 0 && a.conj_dst) {
-#pragma unroll
-    for (int it = 0; it < E; ++it) a.conj_dst[l + TL * it] = R[it];
-  }
-
   // ---------------- INV: x-inverse FFTs of the derived fields ---------------
+  // f = 0: psi_hat = S/k^2, 1: -i kx psi_hat, 2: i kx S, 3: S
+  auto derive = [](int f, double2 s, int kx, int kyv) -> double2 {
+    const int k2i = kx * kx + kyv * kyv;
+    const double inv = k2i ? 1.0 / (double)k2i : 0.0;
+    if (f == 0) return cscale(inv, s);
+    if (f == 1) return make_double2(kx * inv * s.y, -(kx * inv) * s.x);
+    if (f == 2) return make_double2(-kx * s.y, kx * s.x);
+    return s;
+  };
 #pragma unroll 1
   for (int fi = 0; fi < FPS; ++fi) {
     const int f = split * FPS + fi;
     const int l_o = opaque(l), ky_o = opaque(ky);
     double2 F[E];
+    if (pblk) {
+      if (pair) {
+        // Y = P + i Q in F, V = conj P + i conj Q to the line buffer
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int kx = kx_of_row(l_o + TL * e, nx);
-      const int k2i = kx * kx + ky_o * ky_o;
-      const double inv = k2i ? 1.0 / (double)k2i : 0.0;
-      const double2 s = R[e];
-      if (f == 0) F[e] = cscale(inv, s);                                      // psi_hat
-      else if (f == 1) F[e] = make_double2(kx * inv * s.y, -(kx * inv) * s.x);  // -i kx psi_hat
-      else if (f == 2) F[e] = make_double2(-kx * s.y, kx * s.x);              // i kx w_hat
-      else F[e] = s;                                                          // w_hat
+        for (int e = 0; e < E; ++e) {
+          const int x = l_o + TL * e;
+          const int kx = kx_of_row(x, nx);
+          const double2 s0 = hold[x], sh = hold[L + x];
+          double2 P, Q;
+          if (f == 0) {
+            const double2 ps = derive(0, sh, kx, H);
+            P = make_double2(ps.y, -ps.x);
+            Q = make_double2(sh.y, -sh.x);
+          } else {
+            P = derive(f, s0, kx, 0);
+            Q = derive(f, sh, kx, H);
+          }
+          F[e] = make_double2(P.x - Q.y, P.y + Q.x);
+          buf[padx_s<PS>(x)] = make_double2(P.x + Q.y, Q.x - P.y);
+        }
+      }
+      xsync<BLOCK>();
+      if (pair) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const double2 vm = buf[padx_s<PS>((L - (l_o + TL * e)) & (L - 1))];
+          F[e] = make_double2(0.5 * (F[e].x + vm.x), 0.5 * (F[e].y + vm.y));
+        }
+      }
+      xsync<BLOCK>();
+    }
+    if (!pair) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) F[e] = derive(f, R[e], kx_of_row(l_o + TL * e, nx), ky_o);
     }
     fft_reg<L, TL, +1, BLOCK, PS>(F, buf, tw, l_o);
-    if (colv) {
-      double2* __restrict__ gout = a.g[f];
+    double2* __restrict__ gout = a.g[f];
+    if (pair) {
+      double* __restrict__ g4 = reinterpret_cast<double*>(a.g[3]);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const size_t r = (size_t)(l_o + TL * e) * ld;
+        if (f == 0) {
+          gout[r] = make_double2(0.0, 0.0);
+          gout[r + H] = make_double2(0.0, F[e].x);
+          g4[2 * (r + H) + 1] = F[e].y;
+        } else if (f == 3) {
+          gout[r] = make_double2(F[e].x, 0.0);
+          g4[2 * (r + H)] = F[e].y;
+        } else {
+          gout[r] = make_double2(F[e].x, 0.0);
+          gout[r + H] = make_double2(F[e].y, 0.0);
+        }
+      }
+    } else if (colv) {
 #pragma unroll
       for (int e = 0; e < E; ++e) gout[(size_t)(l_o + TL * e) * ld + ky_o] = F[e];
     }
   }
 }
 
-template <int L, int C, int T, int KS, int PS, int OFF>
-__global__ __launch_bounds__(T) void kpass_kernel(KPassArgs a) {
-  kpass_body<L, C, T, KS, PS, OFF>(a, xcd_remap((int)blockIdx.x, (int)gridDim.x));
+// W: minimum waves per SIMD the register allocation must allow (1 = no limit)
+template <int L, int C, int T, int KS, int PS, int OFF, int PK, int W>
+__global__ __launch_bounds__(T, W) void kpass_kernel(KPassArgs a) {
+  kpass_body<L, C, T, KS, PS, OFF, PK>(a, xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
 // All members of an ensemble size class in one launch: work items
 // [m * per_member, (m+1) * per_member) belong to member m, whose arguments
 // are tab[m] (uniform per workgroup, read with scalar loads).
-template <int L, int C, int T, int KS, int PS, int OFF>
-__global__ __launch_bounds__(T) void kpass_batched(const KPassArgs* __restrict__ tab, int per_member) {
+template <int L, int C, int T, int KS, int PS, int OFF, int PK, int W>
+__global__ __launch_bounds__(T, W) void kpass_batched(const KPassArgs* __restrict__ tab, int per_member) {
   const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   const int m = b / per_member;
-  kpass_body<L, C, T, KS, PS, OFF>(tab[m], b - m * per_member);
+  kpass_body<L, C, T, KS, PS, OFF, PK>(tab[m], b - m * per_member);
 }
 
 // ---------------------------------------------------------------------------
@@ -816,69 +930,88 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
 // ---------------------------------------------------------------------------
 // host-side launchers (template dispatch on the transform lengths)
 // ---------------------------------------------------------------------------
-// kpass variants {L, C, T, KS}: L-point columns, C columns per workgroup,
-// T threads, KS workgroups per column group; the first listed for each L is
-// the default.  Rows per thread E = L*C/T.
+// kpass variants {L, C, T, KS, PS, OFF, PK, W}: L-point columns, C column
+// slots per workgroup, T threads, KS workgroups per column group, LDS padding
+// shift PS and line offset OFF, PK = 1 packs the ky = 0 and ky = Ny/2 columns
+// into slot 0, W minimum waves per SIMD; the first listed for each L is the
+// default.  Rows per thread E = L*C/T.  Measured on square grids: up to
+// L = 2048 the unpacked shape is faster (the packed pair's extra exchanges and
+// held columns make slot 0 the last workgroup to finish, and at 2048 its
+// register budget costs occupancy); at 4096, one workgroup per CU, the packed
+// pair removes the odd column group of the last round (-9 %).
 #define NS2D_KPASS_VARIANTS(X) \
-  X(8, 16, 64, 1, 3, 1)       \
-  X(16, 16, 64, 1, 3, 1)      \
-  X(32, 16, 64, 1, 3, 1)      \
-  X(64, 16, 64, 1, 3, 1)      \
-  X(128, 16, 128, 1, 3, 1)    \
-  X(128, 8, 64, 2, 3, 1)      \
-  X(128, 8, 128, 2, 3, 1)     \
-  X(256, 8, 256, 2, 3, 1)     \
-  X(256, 16, 256, 1, 3, 1)    \
-  X(256, 8, 128, 2, 3, 1)     \
-  X(256, 8, 128, 4, 3, 1)     \
-  X(512, 4, 256, 2, 3, 1)     \
-  X(512, 8, 256, 2, 3, 1)     \
-  X(512, 8, 256, 1, 3, 1)     \
-  X(512, 4, 128, 4, 3, 1)     \
-  X(512, 8, 256, 4, 3, 1)     \
-  X(512, 8, 512, 2, 3, 1)     \
-  X(1024, 2, 256, 2, 3, 1)    \
-  X(1024, 2, 256, 2, 3, 4)    \
-  X(1024, 2, 256, 2, 3, 8)    \
-  X(1024, 4, 256, 2, 3, 1)    \
-  X(1024, 4, 256, 1, 3, 1)    \
-  X(1024, 4, 256, 4, 3, 1)    \
-  X(1024, 2, 128, 2, 3, 1)    \
-  X(1024, 2, 128, 4, 3, 1)    \
-  X(1024, 1, 64, 4, 3, 1)     \
-  X(1024, 4, 512, 2, 3, 1)    \
-  X(1024, 4, 512, 1, 3, 1)    \
-  X(1024, 8, 512, 4, 4, 4)    \
-  X(1024, 8, 1024, 4, 3, 4)   \
-  X(1024, 8, 1024, 2, 3, 4)   \
-  X(2048, 2, 512, 1, 3, 1)    \
-  X(2048, 2, 256, 2, 3, 1)    \
-  X(2048, 2, 256, 1, 3, 1)    \
-  X(2048, 4, 512, 1, 3, 1)    \
-  X(2048, 1, 128, 4, 3, 1)    \
-  X(2048, 4, 1024, 1, 3, 1)   \
-  X(4096, 2, 512, 1, 3, 1)    \
-  X(4096, 2, 512, 1, 4, 4)    \
-  X(4096, 2, 512, 1, 4, 8)    \
-  X(4096, 2, 512, 1, 3, 4)    \
-  X(4096, 1, 256, 1, 3, 1)    \
-  X(4096, 1, 256, 2, 3, 1)    \
-  X(4096, 2, 512, 2, 3, 1)    \
-  X(4096, 2, 1024, 1, 3, 1)   \
-  X(4096, 2, 1024, 1, 3, 4)   \
-  X(4096, 2, 1024, 1, 3, 8)   \
-  X(4096, 1, 512, 1, 3, 1)    \
-  X(4096, 1, 512, 2, 3, 1)   
+  X(8, 16, 64, 1, 3, 1, 0, 1) \
+  X(8, 16, 64, 1, 3, 1, 1, 1) \
+  X(16, 16, 64, 1, 3, 1, 0, 1) \
+  X(16, 16, 64, 1, 3, 1, 1, 1) \
+  X(32, 16, 64, 1, 3, 1, 0, 1) \
+  X(32, 16, 64, 1, 3, 1, 1, 1) \
+  X(64, 16, 64, 1, 3, 1, 0, 1) \
+  X(64, 16, 64, 1, 3, 1, 1, 1) \
+  X(128, 16, 128, 1, 3, 1, 0, 1) \
+  X(128, 16, 128, 1, 3, 1, 1, 1) \
+  X(128, 8, 64, 2, 3, 1, 1, 1) \
+  X(128, 8, 128, 2, 3, 1, 1, 1) \
+  X(256, 8, 256, 2, 3, 1, 0, 1) \
+  X(256, 8, 256, 2, 3, 1, 1, 4) \
+  X(256, 16, 256, 1, 3, 1, 1, 1) \
+  X(256, 8, 128, 2, 3, 1, 1, 1) \
+  X(256, 8, 128, 4, 3, 1, 1, 1) \
+  X(512, 4, 256, 2, 3, 1, 0, 1) \
+  X(512, 4, 256, 2, 3, 1, 1, 4) \
+  X(512, 8, 256, 2, 3, 1, 1, 1) \
+  X(512, 8, 256, 1, 3, 1, 1, 1) \
+  X(512, 4, 128, 4, 3, 1, 1, 1) \
+  X(512, 8, 256, 4, 3, 1, 1, 1) \
+  X(512, 8, 512, 2, 3, 1, 1, 1) \
+  X(1024, 2, 256, 2, 3, 1, 0, 1) \
+  X(1024, 2, 256, 2, 3, 1, 1, 1) \
+  X(1024, 2, 256, 2, 3, 1, 1, 4) \
+  X(1024, 2, 256, 2, 3, 4, 1, 1) \
+  X(1024, 2, 256, 2, 3, 8, 1, 1) \
+  X(1024, 4, 256, 2, 3, 1, 1, 1) \
+  X(1024, 4, 256, 1, 3, 1, 1, 1) \
+  X(1024, 4, 256, 4, 3, 1, 1, 1) \
+  X(1024, 2, 128, 2, 3, 1, 1, 1) \
+  X(1024, 2, 128, 4, 3, 1, 1, 1) \
+  X(1024, 1, 64, 4, 3, 1, 1, 1) \
+  X(1024, 4, 512, 2, 3, 1, 1, 1) \
+  X(1024, 4, 512, 1, 3, 1, 1, 1) \
+  X(1024, 8, 512, 4, 4, 4, 1, 1) \
+  X(1024, 8, 1024, 4, 3, 4, 1, 1) \
+  X(1024, 8, 1024, 2, 3, 4, 1, 1) \
+  X(2048, 2, 512, 1, 3, 1, 0, 1) \
+  X(2048, 2, 512, 1, 3, 1, 1, 4) \
+  X(2048, 2, 256, 2, 3, 1, 1, 1) \
+  X(2048, 2, 256, 1, 3, 1, 1, 1) \
+  X(2048, 4, 512, 1, 3, 1, 1, 1) \
+  X(2048, 1, 128, 4, 3, 1, 1, 1) \
+  X(2048, 4, 1024, 1, 3, 1, 1, 1) \
+  X(4096, 2, 512, 1, 3, 1, 1, 1) \
+  X(4096, 2, 512, 1, 4, 4, 1, 1) \
+  X(4096, 2, 512, 1, 4, 8, 1, 1) \
+  X(4096, 2, 512, 1, 3, 4, 1, 1) \
+  X(4096, 1, 256, 1, 3, 1, 1, 1) \
+  X(4096, 1, 256, 2, 3, 1, 1, 1) \
+  X(4096, 2, 512, 2, 3, 1, 1, 1) \
+  X(4096, 2, 1024, 1, 3, 1, 1, 1) \
+  X(4096, 2, 1024, 1, 3, 4, 1, 1) \
+  X(4096, 2, 1024, 1, 3, 8, 1, 1) \
+  X(4096, 1, 512, 1, 3, 1, 1, 1) \
+  X(4096, 1, 512, 2, 3, 1, 1, 1) \
+  X(4096, 2, 512, 1, 3, 1, 0, 1)
 
 struct KVariant {
-  int L, C, T, KS, PS, OFF;
+  int L, C, T, KS, PS, OFF, PK, W;
   void (*kern)(KPassArgs);
   void (*batched)(const KPassArgs*, int);
+  // work items per column group count: the slots cover nyf columns (PK: nyf-1)
+  int groups(int nyf) const { return ((PK ? nyf - 1 : nyf) + C - 1) / C; }
 };
 
-#define NS2D_KV_ENTRY(L_, C_, T_, KS_, PS_, OFF_)                                    \
-  {L_,  C_,  T_, KS_, PS_, OFF_, kpass_kernel<L_, C_, T_, KS_, PS_, OFF_>, \
-   kpass_batched<L_, C_, T_, KS_, PS_, OFF_>},
+#define NS2D_KV_ENTRY(L_, C_, T_, KS_, PS_, OFF_, PK_, W_)                                    \
+  {L_,  C_,  T_, KS_, PS_, OFF_, PK_, W_, kpass_kernel<L_, C_, T_, KS_, PS_, OFF_, PK_, W_>, \
+   kpass_batched<L_, C_, T_, KS_, PS_, OFF_, PK_, W_>},
 static const KVariant kvariants[] = {NS2D_KPASS_VARIANTS(NS2D_KV_ENTRY)};
 #undef NS2D_KV_ENTRY
 
@@ -896,7 +1029,7 @@ static const KVariant* find_kvariant(int nx, int idx) {
 hipError_t launch_kpass_batched(int nx, int nyf, int variant, const KPassArgs* tab, int members, hipStream_t s) {
   const KVariant* v = find_kvariant(nx, variant);
   if (!v) return hipErrorInvalidValue;
-  const int per = (nyf + v->C - 1) / v->C * v->KS;
+  const int per = v->groups(nyf) * v->KS;
   hipLaunchKernelGGL(v->batched, dim3(per * members), dim3(v->T), 0, s, tab, per);
   return hipGetLastError();
 }
@@ -917,12 +1050,13 @@ int kpass_variant_info(int nx, int idx, int* C, int* E, int* T, int* KS) {
   return 0;
 }
 
-int kpass_variant_extra(int nx, int idx, int* W, int* PS, int* OFF) {
+int kpass_variant_extra(int nx, int idx, int* W, int* PS, int* OFF, int* PK) {
   const KVariant* v = find_kvariant(nx, idx);
   if (!v) return -1;
-  *W = 1;
+  *W = v->W;
   *PS = v->PS;
   *OFF = v->OFF;
+  *PK = v->PK;
   return 0;
 }
 
@@ -935,7 +1069,8 @@ hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s) 
   if (!(a.mode & KP_INV) && v->KS != 1) {
     const KVariant* w = nullptr;
     for (const KVariant& u : kvariants)
-      if (u.L == nx && u.KS == 1 && u.C == v->C && u.T == v->T && u.PS == v->PS && u.OFF == v->OFF) {
+      if (u.L == nx && u.KS == 1 && u.C == v->C && u.T == v->T && u.PS == v->PS && u.OFF == v->OFF &&
+          u.PK == v->PK && u.W == v->W) {
         w = &u;
         break;
       }
@@ -945,7 +1080,7 @@ hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s) 
     if (!w) return hipErrorInvalidValue;
     v = w;
   }
-  const int groups = (a.nyf + v->C - 1) / v->C;
+  const int groups = v->groups(a.nyf);
   hipLaunchKernelGGL(v->kern, dim3(groups * v->KS), dim3(v->T), 0, s, a);
   return hipGetLastError();
 }
@@ -1026,12 +1161,13 @@ int rpass_variant_info(int ny, int idx, int* TLR, int* E, int* T, int* PAIRS) {
   return 0;
 }
 
-int rpass_variant_extra(int ny, int idx, int* W, int* PS, int* OFF) {
+int rpass_variant_extra(int ny, int idx, int* W, int* PS, int* OFF, int* PK) {
   const RVariant* v = find_rvariant(ny, idx);
   if (!v) return -1;
   *W = v->W;
   *PS = 3;
   *OFF = 1;
+  *PK = 0;
   return 0;
 }
 

@@ -169,6 +169,59 @@ def _variant_cases():
     return cases
 
 
+def _pair_cases():
+    import ns2d
+    try:
+        ns2d.load_library()
+    except OSError:
+        return []
+    cases = []
+    for nx, ny in [(64, 32), (256, 64), (512, 128), (1024, 32), (2048, 16), (4096, 8), (128, 8)]:
+        vs = ns2d.kpass_variants(nx)
+        packed = [i for i, v in enumerate(vs) if v["packed_pair"]]
+        unpacked = [i for i, v in enumerate(vs) if not v["packed_pair"]]
+        cases.append((nx, ny, packed[0], unpacked[0]))
+    return cases
+
+
+@pytest.mark.parametrize("Nx,Ny,packed,unpacked", _pair_cases())
+@pytest.mark.parametrize("filt", ["none", "houli"])
+def test_packed_pair_column_pass(Nx, Ny, packed, unpacked, filt):
+    """Column-pass shapes that carry ky = 0 and ky = Ny/2 as one packed
+    complex line, against the oracle and an unpacked shape.  With no filter
+    (or the smooth Hou-Li one) the Nyquist column holds data, and the ky = 0
+    column is made non-Hermitian.  Packing shares rounding between the two
+    columns (each column's error scales with the larger of the two), so the
+    Nyquist column here is 1 % of the largest mode and the packed/unpacked
+    agreement bound is 1e-12."""
+    print(f"packed pair {Nx}x{Ny} {filt}", flush=True)
+    p = O.Params(Nx, Ny, nu=1e-3, hyper_flag=0, hyper_pow=1.0, filter=filt)
+    rng = np.random.default_rng(Nx + 3 * Ny)
+    w0 = O.ic_random_spectrum(p, seed=Nx + Ny, k0=3.0)
+    scale = np.abs(w0).max()
+    w0[:, -1] = 0.01 * scale * (rng.standard_normal(Nx) + 1j * rng.standard_normal(Nx))
+    w0[Nx - 1, 0] += 0.2 * scale
+    w0[0, 0] = 0.0
+    st = O.State(w0, p)
+    O.step(st, 1e-4, 3)
+    x = rng.standard_normal((Nx, Ny))
+    out = {}
+    for var in (packed, unpacked):
+        with solver(p) as s:
+            s.set_kpass_variant(var)
+            s.set_state(w0)
+            s.spectral_step(1e-4, 3)
+            out[var] = (s.get_state(), s.nonlinear_rhs(w0), s.r2c(x), s.c2r(w0))
+    for var in (packed, unpacked):
+        got, K, xf, wr = out[var]
+        assert rel(got, st.w_hat) < 1e-12
+        assert rel(K, O.nonlinear_rhs(w0, p)) < 1e-12
+        assert rel(xf, O.r2c(x)) < 1e-13
+        assert rel(wr, O.c2r(w0, Nx, Ny)) < 1e-13
+    for a, b in zip(out[packed], out[unpacked]):
+        assert rel(a, b) < 1e-12
+
+
 @pytest.mark.parametrize("Nx,Ny,var,rvar", _variant_cases())
 @pytest.mark.parametrize("integ", ["RK4CN", "AB4"])
 def test_pass_variants_match_oracle(Nx, Ny, var, rvar, integ):
