@@ -1,6 +1,7 @@
 """Builds, in-tree: libns2d_gpu.so (HIP kernels + C ABI) for gfx950 with hipcc;
 libns2d_host.so (command line, initial conditions, HDF5 output; plain C) and
-the ns2d_solver executable (the reference-compatible driver) with gcc."""
+the ns2d_solver executable (the reference-compatible driver) with gcc; the
+probes/ measurement programs with hipcc."""
 import os
 import subprocess
 import sys
@@ -39,7 +40,17 @@ def build(verbose: bool = False) -> str:
     for o in objs:
         os.remove(o)
     build_driver(verbose)
+    build_probes(verbose)
     return LIB
+
+
+def build_probes(verbose: bool = False) -> None:
+    """Stand-alone measurement programs under probes/ (not part of the library)."""
+    src = os.path.join(HERE, "probes", "column_bandwidth.hip")
+    cmd = [HIPCC] + FLAGS + [src, "-o", os.path.splitext(src)[0]]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
 
 
 def build_driver(verbose: bool = False) -> str:

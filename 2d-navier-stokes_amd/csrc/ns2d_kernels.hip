@@ -108,7 +108,7 @@ __device__ __forceinline__ double dealias_factor(const PhysArgs& ph, int kx, int
 // PS: LDS padding (one slot per 2^PS elements); OFF: extra 16-B slots between
 // the slots' lines, which sets where the neighbouring slot's accesses of one
 // wave fall among the LDS banks (adjacent lanes work on adjacent slots)
-template <int L, int C, int T, int KS, int PS, int OFF, int PK>
+template <int L, int C, int T, int KS, int PS, int OFF, int PK, int PF>
 __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
   constexpr int TL = T / C;        // threads per column
   constexpr int E = L / TL;        // rows per thread
@@ -135,6 +135,34 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
   double2* __restrict__ hold = a.pairbuf + (size_t)split * 2 * L;
 
   double2 R[E];  // forward result K, then the next stage input S
+
+  // PF: the RK epilogue's state, accumulator and coefficient loads are issued
+  // before the forward transform, so that their latency hides behind it
+  // (unpacked shapes whose register budget leaves room for 2.5 E values)
+  static_assert(!(PF && PK), "prefetch is for unpacked shapes");
+  constexpr int EP = PF ? E : 1;
+  double2 wpre[EP], apre[EP];
+  double lpre[EP];
+  // (every epilogue that reads the state runs after a forward transform on a
+  // valid column, exactly when `pre` holds; the arrays are filled
+  // unconditionally so that they stay in registers)
+  const bool pre = PF && (a.mode & KP_FWD) && colv;
+  if constexpr (PF != 0) {
+    {
+      const bool cf = (a.mode & KP_CONJFIX) && ky == 0;
+      const bool need_acc = a.epi == EPI_RK4CN || ((a.epi == EPI_RK2 || a.epi == EPI_RK3) && writer);
+      const bool need_lam = a.epi == EPI_RK4CN;
+#pragma unroll
+      for (int it = 0; it < E; ++it) {
+        const int i = l + TL * it;
+        const int kx = kx_of_row(i, nx);
+        const size_t idx = (size_t)i * ld + ky;
+        wpre[it] = !pre ? make_double2(0.0, 0.0) : (cf && kx < 0) ? cconj(a.conj_src[-kx]) : a.w_in[idx];
+        apre[it] = (pre && need_acc) ? a.acc[idx] : make_double2(0.0, 0.0);
+        lpre[it] = (pre && need_lam) ? a.lam[idx] : 0.0;
+      }
+    }
+  }
 
   // ---------------- FWD: x-forward FFT of the row-pass output -------------
   if (a.mode & KP_FWD) {
@@ -201,10 +229,19 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
     }
     const bool conj_fix = (a.mode & KP_CONJFIX) && kq == 0;
     auto state_at = [&](int it) -> double2 {
+      if constexpr (PF != 0) return wpre[it];
       const int i = lq + TL * it;
       const int kx = kx_of_row(i, nx);
       if (conj_fix && kx < 0) return cconj(a.conj_src[-kx]);
       return a.w_in[(size_t)i * ld + kq];
+    };
+    auto acc_at = [&](int it) -> double2 {
+      if constexpr (PF != 0) return apre[it];
+      return a.acc[index_of(it)];
+    };
+    auto lam_at = [&](int it) -> double {
+      if constexpr (PF != 0) return lpre[it];
+      return a.lam[index_of(it)];
     };
     switch (a.epi) {
       case EPI_FROM_STATE:
@@ -252,7 +289,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
 #pragma unroll
           for (int it = 0; it < E; ++it) {
             const double2 k = R[it];
-            if (writer) a.acc[index_of(it)] = cadd(a.acc[index_of(it)], cscale(b, k));
+            if (writer) a.acc[index_of(it)] = cadd(acc_at(it), cscale(b, k));
             R[it] = cadd(state_at(it), cscale(a.coef, k));     // solver.c:729,768
             if ((it & 3) == 3) asm volatile("" ::: "memory");
           }
@@ -265,8 +302,8 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
             const int kx = kx_of_row(lq + TL * it, nx);
             double2 nw = make_double2(0.0, 0.0);               // (0,0) mode: solver.c:860
             if (kx != 0 || kq != 0) {
-              const double2 sum = cadd(a.acc[index_of(it)], cscale(RK4_B4, R[it]));
-              const double D = a.dt * a.lam[index_of(it)];
+              const double2 sum = cadd(acc_at(it), cscale(RK4_B4, R[it]));
+              const double D = a.dt * lam_at(it);
               nw = cadd(cscale((2.0 - D) / (2.0 + D), state_at(it)), cscale(2.0 * a.dt / (2.0 + D), sum));  // :853
             }
             if (writer) a.w_out[index_of(it)] = nw;
@@ -408,19 +445,19 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
 }
 
 // W: minimum waves per SIMD the register allocation must allow (1 = no limit)
-template <int L, int C, int T, int KS, int PS, int OFF, int PK, int W>
+template <int L, int C, int T, int KS, int PS, int OFF, int PK, int W, int PF>
 __global__ __launch_bounds__(T, W) void kpass_kernel(KPassArgs a) {
-  kpass_body<L, C, T, KS, PS, OFF, PK>(a, xcd_remap((int)blockIdx.x, (int)gridDim.x));
+  kpass_body<L, C, T, KS, PS, OFF, PK, PF>(a, xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
 // All members of an ensemble size class in one launch: work items
 // [m * per_member, (m+1) * per_member) belong to member m, whose arguments
 // are tab[m] (uniform per workgroup, read with scalar loads).
-template <int L, int C, int T, int KS, int PS, int OFF, int PK, int W>
+template <int L, int C, int T, int KS, int PS, int OFF, int PK, int W, int PF>
 __global__ __launch_bounds__(T, W) void kpass_batched(const KPassArgs* __restrict__ tab, int per_member) {
   const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   const int m = b / per_member;
-  kpass_body<L, C, T, KS, PS, OFF, PK>(tab[m], b - m * per_member);
+  kpass_body<L, C, T, KS, PS, OFF, PK, PF>(tab[m], b - m * per_member);
 }
 
 // ---------------------------------------------------------------------------
@@ -930,88 +967,93 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
 // ---------------------------------------------------------------------------
 // host-side launchers (template dispatch on the transform lengths)
 // ---------------------------------------------------------------------------
-// kpass variants {L, C, T, KS, PS, OFF, PK, W}: L-point columns, C column
+// kpass variants {L, C, T, KS, PS, OFF, PK, W, PF}: L-point columns, C column
 // slots per workgroup, T threads, KS workgroups per column group, LDS padding
 // shift PS and line offset OFF, PK = 1 packs the ky = 0 and ky = Ny/2 columns
-// into slot 0, W minimum waves per SIMD; the first listed for each L is the
+// into slot 0, W minimum waves per SIMD, PF = 1 issues the RK epilogue's
+// loads before the forward transform; the first listed for each L is the
 // default.  Rows per thread E = L*C/T.  Measured on square grids: up to
 // L = 2048 the unpacked shape is faster (the packed pair's extra exchanges and
 // held columns make slot 0 the last workgroup to finish, and at 2048 its
 // register budget costs occupancy); at 4096, one workgroup per CU, the packed
 // pair removes the odd column group of the last round (-9 %).
 #define NS2D_KPASS_VARIANTS(X) \
-  X(8, 16, 64, 1, 3, 1, 0, 1) \
-  X(8, 16, 64, 1, 3, 1, 1, 1) \
-  X(16, 16, 64, 1, 3, 1, 0, 1) \
-  X(16, 16, 64, 1, 3, 1, 1, 1) \
-  X(32, 16, 64, 1, 3, 1, 0, 1) \
-  X(32, 16, 64, 1, 3, 1, 1, 1) \
-  X(64, 16, 64, 1, 3, 1, 0, 1) \
-  X(64, 16, 64, 1, 3, 1, 1, 1) \
-  X(128, 16, 128, 1, 3, 1, 0, 1) \
-  X(128, 16, 128, 1, 3, 1, 1, 1) \
-  X(128, 8, 64, 2, 3, 1, 1, 1) \
-  X(128, 8, 128, 2, 3, 1, 1, 1) \
-  X(256, 8, 256, 2, 3, 1, 0, 1) \
-  X(256, 8, 256, 2, 3, 1, 1, 4) \
-  X(256, 16, 256, 1, 3, 1, 1, 1) \
-  X(256, 8, 128, 2, 3, 1, 1, 1) \
-  X(256, 8, 128, 4, 3, 1, 1, 1) \
-  X(512, 4, 256, 2, 3, 1, 0, 1) \
-  X(512, 4, 256, 2, 3, 1, 1, 4) \
-  X(512, 8, 256, 2, 3, 1, 1, 1) \
-  X(512, 8, 256, 1, 3, 1, 1, 1) \
-  X(512, 4, 128, 4, 3, 1, 1, 1) \
-  X(512, 8, 256, 4, 3, 1, 1, 1) \
-  X(512, 8, 512, 2, 3, 1, 1, 1) \
-  X(1024, 2, 256, 2, 3, 1, 0, 1) \
-  X(1024, 2, 256, 2, 3, 1, 1, 1) \
-  X(1024, 2, 256, 2, 3, 1, 1, 4) \
-  X(1024, 2, 256, 2, 3, 4, 1, 1) \
-  X(1024, 2, 256, 2, 3, 8, 1, 1) \
-  X(1024, 4, 256, 2, 3, 1, 1, 1) \
-  X(1024, 4, 256, 1, 3, 1, 1, 1) \
-  X(1024, 4, 256, 4, 3, 1, 1, 1) \
-  X(1024, 2, 128, 2, 3, 1, 1, 1) \
-  X(1024, 2, 128, 4, 3, 1, 1, 1) \
-  X(1024, 1, 64, 4, 3, 1, 1, 1) \
-  X(1024, 4, 512, 2, 3, 1, 1, 1) \
-  X(1024, 4, 512, 1, 3, 1, 1, 1) \
-  X(1024, 8, 512, 4, 4, 4, 1, 1) \
-  X(1024, 8, 1024, 4, 3, 4, 1, 1) \
-  X(1024, 8, 1024, 2, 3, 4, 1, 1) \
-  X(2048, 2, 512, 1, 3, 1, 0, 1) \
-  X(2048, 2, 512, 1, 3, 1, 1, 4) \
-  X(2048, 2, 256, 2, 3, 1, 1, 1) \
-  X(2048, 2, 256, 1, 3, 1, 1, 1) \
-  X(2048, 4, 512, 1, 3, 1, 1, 1) \
-  X(2048, 1, 128, 4, 3, 1, 1, 1) \
-  X(2048, 4, 1024, 1, 3, 1, 1, 1) \
-  X(4096, 2, 512, 1, 3, 1, 1, 1) \
-  X(4096, 2, 512, 1, 4, 4, 1, 1) \
-  X(4096, 2, 512, 1, 4, 8, 1, 1) \
-  X(4096, 2, 512, 1, 3, 4, 1, 1) \
-  X(4096, 1, 256, 1, 3, 1, 1, 1) \
-  X(4096, 1, 256, 2, 3, 1, 1, 1) \
-  X(4096, 2, 512, 2, 3, 1, 1, 1) \
-  X(4096, 2, 1024, 1, 3, 1, 1, 1) \
-  X(4096, 2, 1024, 1, 3, 4, 1, 1) \
-  X(4096, 2, 1024, 1, 3, 8, 1, 1) \
-  X(4096, 1, 512, 1, 3, 1, 1, 1) \
-  X(4096, 1, 512, 2, 3, 1, 1, 1) \
-  X(4096, 2, 512, 1, 3, 1, 0, 1)
+  X(8, 16, 64, 1, 3, 1, 0, 1, 0) \
+  X(8, 16, 64, 1, 3, 1, 1, 1, 0) \
+  X(16, 16, 64, 1, 3, 1, 0, 1, 0) \
+  X(16, 16, 64, 1, 3, 1, 1, 1, 0) \
+  X(32, 16, 64, 1, 3, 1, 0, 1, 0) \
+  X(32, 16, 64, 1, 3, 1, 1, 1, 0) \
+  X(64, 16, 64, 1, 3, 1, 0, 1, 0) \
+  X(64, 16, 64, 1, 3, 1, 1, 1, 0) \
+  X(128, 16, 128, 1, 3, 1, 0, 1, 0) \
+  X(128, 16, 128, 1, 3, 1, 1, 1, 0) \
+  X(128, 8, 64, 2, 3, 1, 1, 1, 0) \
+  X(128, 8, 128, 2, 3, 1, 1, 1, 0) \
+  X(256, 8, 256, 2, 3, 1, 0, 1, 0) \
+  X(256, 8, 256, 2, 3, 1, 0, 1, 1) \
+  X(256, 8, 256, 2, 3, 1, 1, 4, 0) \
+  X(256, 16, 256, 1, 3, 1, 1, 1, 0) \
+  X(256, 8, 128, 2, 3, 1, 1, 1, 0) \
+  X(256, 8, 128, 4, 3, 1, 1, 1, 0) \
+  X(512, 4, 256, 2, 3, 1, 0, 1, 0) \
+  X(512, 4, 256, 2, 3, 1, 0, 1, 1) \
+  X(512, 4, 256, 2, 3, 1, 1, 4, 0) \
+  X(512, 8, 256, 2, 3, 1, 1, 1, 0) \
+  X(512, 8, 256, 1, 3, 1, 1, 1, 0) \
+  X(512, 4, 128, 4, 3, 1, 1, 1, 0) \
+  X(512, 8, 256, 4, 3, 1, 1, 1, 0) \
+  X(512, 8, 512, 2, 3, 1, 1, 1, 0) \
+  X(1024, 2, 256, 2, 3, 1, 0, 1, 0) \
+  X(1024, 2, 256, 2, 3, 1, 0, 1, 1) \
+  X(1024, 2, 256, 2, 3, 1, 1, 1, 0) \
+  X(1024, 2, 256, 2, 3, 1, 1, 4, 0) \
+  X(1024, 2, 256, 2, 3, 4, 1, 1, 0) \
+  X(1024, 2, 256, 2, 3, 8, 1, 1, 0) \
+  X(1024, 4, 256, 2, 3, 1, 1, 1, 0) \
+  X(1024, 4, 256, 1, 3, 1, 1, 1, 0) \
+  X(1024, 4, 256, 4, 3, 1, 1, 1, 0) \
+  X(1024, 2, 128, 2, 3, 1, 1, 1, 0) \
+  X(1024, 2, 128, 4, 3, 1, 1, 1, 0) \
+  X(1024, 1, 64, 4, 3, 1, 1, 1, 0) \
+  X(1024, 4, 512, 2, 3, 1, 1, 1, 0) \
+  X(1024, 4, 512, 1, 3, 1, 1, 1, 0) \
+  X(1024, 8, 512, 4, 4, 4, 1, 1, 0) \
+  X(1024, 8, 1024, 4, 3, 4, 1, 1, 0) \
+  X(1024, 8, 1024, 2, 3, 4, 1, 1, 0) \
+  X(2048, 2, 512, 1, 3, 1, 0, 1, 0) \
+  X(2048, 2, 512, 1, 3, 1, 0, 1, 1) \
+  X(2048, 2, 512, 1, 3, 1, 1, 4, 0) \
+  X(2048, 2, 256, 2, 3, 1, 1, 1, 0) \
+  X(2048, 2, 256, 1, 3, 1, 1, 1, 0) \
+  X(2048, 4, 512, 1, 3, 1, 1, 1, 0) \
+  X(2048, 1, 128, 4, 3, 1, 1, 1, 0) \
+  X(2048, 4, 1024, 1, 3, 1, 1, 1, 0) \
+  X(4096, 2, 512, 1, 3, 1, 1, 1, 0) \
+  X(4096, 2, 512, 1, 4, 4, 1, 1, 0) \
+  X(4096, 2, 512, 1, 4, 8, 1, 1, 0) \
+  X(4096, 2, 512, 1, 3, 4, 1, 1, 0) \
+  X(4096, 1, 256, 1, 3, 1, 1, 1, 0) \
+  X(4096, 1, 256, 2, 3, 1, 1, 1, 0) \
+  X(4096, 2, 512, 2, 3, 1, 1, 1, 0) \
+  X(4096, 2, 1024, 1, 3, 1, 1, 1, 0) \
+  X(4096, 2, 1024, 1, 3, 4, 1, 1, 0) \
+  X(4096, 2, 1024, 1, 3, 8, 1, 1, 0) \
+  X(4096, 1, 512, 1, 3, 1, 1, 1, 0) \
+  X(4096, 1, 512, 2, 3, 1, 1, 1, 0) \
+  X(4096, 2, 512, 1, 3, 1, 0, 1, 0)
 
 struct KVariant {
-  int L, C, T, KS, PS, OFF, PK, W;
+  int L, C, T, KS, PS, OFF, PK, W, PF;
   void (*kern)(KPassArgs);
   void (*batched)(const KPassArgs*, int);
   // work items per column group count: the slots cover nyf columns (PK: nyf-1)
   int groups(int nyf) const { return ((PK ? nyf - 1 : nyf) + C - 1) / C; }
 };
 
-#define NS2D_KV_ENTRY(L_, C_, T_, KS_, PS_, OFF_, PK_, W_)                                    \
-  {L_,  C_,  T_, KS_, PS_, OFF_, PK_, W_, kpass_kernel<L_, C_, T_, KS_, PS_, OFF_, PK_, W_>, \
-   kpass_batched<L_, C_, T_, KS_, PS_, OFF_, PK_, W_>},
+#define NS2D_KV_ENTRY(L_, C_, T_, KS_, PS_, OFF_, PK_, W_, PF_)                                    \
+  {L_,  C_,  T_, KS_, PS_, OFF_, PK_, W_, PF_, kpass_kernel<L_, C_, T_, KS_, PS_, OFF_, PK_, W_, PF_>, \
+   kpass_batched<L_, C_, T_, KS_, PS_, OFF_, PK_, W_, PF_>},
 static const KVariant kvariants[] = {NS2D_KPASS_VARIANTS(NS2D_KV_ENTRY)};
 #undef NS2D_KV_ENTRY
 
@@ -1056,7 +1098,7 @@ int kpass_variant_extra(int nx, int idx, int* W, int* PS, int* OFF, int* PK) {
   *W = v->W;
   *PS = v->PS;
   *OFF = v->OFF;
-  *PK = v->PK;
+  *PK = v->PK | (v->PF << 1);
   return 0;
 }
 
@@ -1070,7 +1112,7 @@ hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s) 
     const KVariant* w = nullptr;
     for (const KVariant& u : kvariants)
       if (u.L == nx && u.KS == 1 && u.C == v->C && u.T == v->T && u.PS == v->PS && u.OFF == v->OFF &&
-          u.PK == v->PK && u.W == v->W) {
+          u.PK == v->PK && u.W == v->W && u.PF == v->PF) {
         w = &u;
         break;
       }

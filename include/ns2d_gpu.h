@@ -169,9 +169,11 @@ int ns2d_variant_count(int pass, int n);
 int ns2d_variant_info(int pass, int n, int idx, int* shape4);
 /* Further compile-time choices of a shape: extra4 = {minimum waves per SIMD
  * its register budget allows (1: no constraint), LDS padding shift (one slot
- * per 2^shift elements), 16-B slots between neighbouring lines, 1 if the
- * column pass carries the ky = 0 and ky = Ny/2 columns as one packed complex
- * line (Ny/2 column slots instead of Ny/2+1; 0 for the row pass)}. */
+ * per 2^shift elements), 16-B slots between neighbouring lines, flags}; flag
+ * bit 0: the column pass carries the ky = 0 and ky = Ny/2 columns as one
+ * packed complex line (Ny/2 column slots instead of Ny/2+1); bit 1: the
+ * column pass issues its RK epilogue loads before the forward transform
+ * (flags are 0 for the row pass). */
 int ns2d_variant_extra(int pass, int n, int idx, int* extra4);
 int ns2d_set_variant(ns2d_t* h, int pass, int idx);
 
