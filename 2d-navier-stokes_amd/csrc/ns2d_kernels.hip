@@ -56,10 +56,11 @@ __device__ __forceinline__ double dealias_factor(const PhysArgs& ph, int kx, int
       if (ph.kolm_kill && kx == 0) return 0.0;
       return 1.0;
     case FILTER_HOULI_REF: {
-      // integer division k/(N/2) (truncating): only the Nyquist row/column
-      // differ from 1; the two possible values come precomputed from the host
-      const int qx = kx / (ph.nx / 2), qy = ky / (ph.ny / 2);
-      const int q = qx * qx + qy * qy;
+      // integer division k/(N/2) (truncating): with kx in (-Nx/2, Nx/2] and
+      // ky in [0, Ny/2] the quotients are 1 on the Nyquist row/column and 0
+      // elsewhere, so only those differ from 1; the two possible values come
+      // precomputed from the host
+      const int q = (kx == ph.nx / 2) + (ky == ph.ny / 2);
       return q == 0 ? 1.0 : (q == 1 ? ph.houli_edge : ph.houli_corner);
     }
     case FILTER_HOULI:   // exp(-36 (|k|/k_N)^36), tabulated on the host

@@ -9,8 +9,9 @@ reference's integer-division form, RK4CN, dt = 5e-4, with the reference's
 per-step max|w| blow-up check.  A "step" is one full RK4CN time step (four
 nonlinear right-hand sides).  The state is resident in HBM before timing
 starts.  Reported under "extra": the same step on a 4096^2 grid, the
-64-member uneven-size ensemble against sequential plain solves, and the
-discrete adjoint's cost relative to a forward solve (512^2, 2000 steps).
+forced Kolmogorov CN-AB4 step on a 4096^2 grid, the 64-member uneven-size
+ensemble against sequential plain solves, and the discrete adjoint's cost
+relative to a forward solve (512^2, 2000 steps).
 
 The roofline object covers the dominant kernel, timed live with HIP events
 recorded on the solver's own stream around every launch; its algorithmic
@@ -173,6 +174,33 @@ def adjoint_leg(n=512, steps=2000, ckpt=100, seed=5):
             "grad_norm": float(np.linalg.norm(g))}
 
 
+def kolmogorov_leg(n=4096, steps=40, warmup=5, seed=3):
+    """Forced Kolmogorov flow at 4096^2: f = F sin(k_f y), k_f = 4, F = 1,
+    linear drag alpha = 0.01, nu = 2e-5, 2/3 filter, CN-AB4 after 3 RK4CN
+    start-up steps (inside the warm-up), dt = 2.5e-4, per-step max|w| check;
+    low-amplitude (E = 1e-6) uniform-phase vorticity on 0 < |k| <= 16."""
+    import ns2d
+    from ns2d import ics
+    cfg = ns2d.Config(Nx=n, Ny=n, nu=2e-5, hyper_flag=0, hyper_pow=1.0, alpha=0.01, drag_flag=1, drag_pow=0.0,
+                      filter="23", integrator="AB4", forcing="KOLM", force_k=4, force_scale=1.0, check_every=1)
+    w0 = ics.random_phase(n, n, seed=seed, e0=1e-6, filt="23", spectrum="flat", kmax=16.0)
+    dt = 2.5e-4
+    with ns2d.Solver(cfg) as s:
+        s.set_state(w0)
+        s.spectral_step(dt, warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        s.spectral_step(dt, steps)
+        s.sync()
+        el = time.perf_counter() - t0
+        kms, rms, _ = s.profile_steps(dt, 10)
+    sps = steps / el
+    M = n * (n // 2 + 1)
+    return {"kolm_ab4_4096_steps_per_s": sps, "kolm_ab4_4096_ms_per_step": 1e3 / sps,
+            "kolm_ab4_4096_kpass_avg_ms": kms, "kolm_ab4_4096_rpass_avg_ms": rms,
+            "kolm_ab4_4096_model272_frac": 272.0 * M * sps / 1e9 / HBM_PEAK_GBS}
+
+
 def cpu_baseline(nx, ny, budget_s=20.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ns2d_oracle as O
@@ -241,6 +269,7 @@ def main():
                       "rk4_4096_kpass_avg_ms": k4, "rk4_4096_rpass_avg_ms": r4,
                       "rk4_4096_step_alg_GBps": ab4["step"] * s4 / 1e9,
                       "rk4_4096_step_model928_frac": ab4["step_model"] * s4 / 1e9 / HBM_PEAK_GBS})
+        extra.update(kolmogorov_leg())
     if not args.no_ensemble:
         extra.update(ensemble_leg())
     if not args.no_adjoint:
