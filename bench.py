@@ -270,6 +270,10 @@ def main():
                       "rk4_4096_step_alg_GBps": ab4["step"] * s4 / 1e9,
                       "rk4_4096_step_model928_frac": ab4["step_model"] * s4 / 1e9 / HBM_PEAK_GBS})
         extra.update(kolmogorov_leg())
+        if not args.no_cpu:
+            # the same RK4CN workload at 4096^2 on the host cores, a few steps
+            c4 = cpu_baseline(4096, 4096, budget_s=12.0)
+            extra.update({"cpu_4096": c4, "rk4_4096_vs_cpu": s4 / c4["value"]})
     if not args.no_ensemble:
         extra.update(ensemble_leg())
     if not args.no_adjoint:
