@@ -15,7 +15,7 @@ relative to a forward solve (512^2, 2000 steps).
 
 The roofline object covers the dominant kernel, timed live with HIP events
 recorded on the solver's own stream around every launch; its algorithmic
-bytes per launch are stated in ALG_BYTES below.  The CPU baseline is the
+bytes per launch are stated in alg_bytes() below.  The CPU baseline is the
 numpy/scipy restatement of the reference algorithm (oracle/) on the host
 cores, timed on a bounded number of steps of the same workload.
 """
@@ -44,8 +44,8 @@ def alg_bytes(nx: int, ny: int) -> dict:
     kpass: reads N and w_hat, reads/writes the RK accumulator, writes the four
            G's (stage 1: acc write only; stage 4: acc read + w_hat write)
            -> (112 + 128 + 128 + 128)/4 * M on average over a step
-    step:  4 rpass + 4 kpass = 816*M;  the two-pass model of the survey
-           (separate inverse and forward column passes) is 928*M.
+    step:  4 rpass + 4 kpass = 816*M;  the byte model with separate inverse
+           and forward column passes is 928*M.
     """
     M = nx * (ny // 2 + 1)
     return {"rpass": 80.0 * M, "kpass": 124.0 * M, "step": 816.0 * M, "step_model": 928.0 * M}
