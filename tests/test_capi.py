@@ -30,6 +30,13 @@ def test_variant_tables_cover_every_size():
         for v in vs:
             assert v["threads"] % v["cols"] == 0 and 4 % v["splits"] == 0
             assert v["per_thread"] * v["threads"] == n * v["cols"]
+        # every length keeps an unpacked shape (the packed pair is an option),
+        # the prefetching epilogue only on unpacked shapes, defaults unpacked
+        # below 4096 (measured) and packed at 4096 (1024 instead of 1025 work items)
+        assert any(not v["packed_pair"] for v in vs), n
+        assert not any(v["packed_pair"] and v["prefetch"] for v in vs), n
+        assert vs[0]["packed_pair"] == (1 if n == 4096 else 0), n
+        assert all(v["min_waves"] >= 1 and v["pad_shift"] >= 3 for v in vs), n
         rs = ns2d.rpass_variants(n)
         assert rs, n
         for v in rs:
