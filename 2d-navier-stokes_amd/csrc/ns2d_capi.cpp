@@ -1087,12 +1087,27 @@ int ens_autotune(ns2d_ensemble* e, int reps) {
     }
     return best;
   };
+  // each shape is timed over the four RK4CN stages of one step (the stage
+  // epilogues read and write different arrays); the timing launches write only
+  // scratch (accumulator, G fields, the next state buffer, the ky=0 hand-off),
+  // which the rebuild below restores
   for (auto& c : e->cls) {
     const int nm = (int)c.idx.size();
+    auto step_ms = [&](auto&& launch_stage) -> float {
+      float sum = 0.0f;
+      for (int st = 0; st < 4; ++st) {
+        const float ms = time_it([&] { return launch_stage(st); });
+        if (ms < 0.0f) return -1.0f;
+        sum += ms;
+      }
+      return sum;
+    };
     float best = 1e30f;
     int arg = c.rvar;
     for (int v = 0; v < rpass_variant_count(c.ny); ++v) {
-      const float ms = time_it([&] { return launch_rpass_batched(c.nx, c.ny, v, c.dr, nm, e->stream); });
+      const float ms = step_ms([&](int st) {
+        return launch_rpass_batched(c.nx, c.ny, v, c.dr + (size_t)st * nm, nm, e->stream);
+      });
       if (ms >= 0.0f && ms < best) {
         best = ms;
         arg = v;
@@ -1102,7 +1117,9 @@ int ens_autotune(ns2d_ensemble* e, int reps) {
     best = 1e30f;
     arg = c.kvar;
     for (int v = 0; v < kpass_variant_count(c.nx); ++v) {
-      const float ms = time_it([&] { return launch_kpass_batched(c.nx, c.nyf, v, c.dk, nm, e->stream); });
+      const float ms = step_ms([&](int st) {
+        return launch_kpass_batched(c.nx, c.nyf, v, c.dk + (size_t)st * nm, nm, e->stream);
+      });
       if (ms >= 0.0f && ms < best) {
         best = ms;
         arg = v;

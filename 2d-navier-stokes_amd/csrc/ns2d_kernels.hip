@@ -997,6 +997,8 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(256, 16, 256, 1, 3, 1, 1, 1, 0) \
   X(256, 8, 128, 2, 3, 1, 1, 1, 0) \
   X(256, 8, 128, 4, 3, 1, 1, 1, 0) \
+  X(256, 16, 256, 2, 3, 1, 0, 1, 0) \
+  X(256, 32, 512, 2, 3, 1, 0, 1, 0) \
   X(512, 4, 256, 2, 3, 1, 0, 1, 0) \
   X(512, 4, 256, 2, 3, 1, 0, 1, 1) \
   X(512, 4, 256, 2, 3, 1, 1, 4, 0) \
@@ -1005,6 +1007,9 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(512, 4, 128, 4, 3, 1, 1, 1, 0) \
   X(512, 8, 256, 4, 3, 1, 1, 1, 0) \
   X(512, 8, 512, 2, 3, 1, 1, 1, 0) \
+  X(512, 8, 512, 2, 3, 1, 0, 1, 0) \
+  X(512, 8, 512, 1, 3, 1, 0, 1, 0) \
+  X(512, 16, 512, 2, 3, 1, 0, 1, 0) \
   X(1024, 2, 256, 2, 3, 1, 0, 1, 0) \
   X(1024, 2, 256, 2, 3, 1, 0, 1, 1) \
   X(1024, 2, 256, 2, 3, 1, 1, 1, 0) \
@@ -1022,6 +1027,9 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(1024, 8, 512, 4, 4, 4, 1, 1, 0) \
   X(1024, 8, 1024, 4, 3, 4, 1, 1, 0) \
   X(1024, 8, 1024, 2, 3, 4, 1, 1, 0) \
+  X(1024, 4, 256, 1, 3, 1, 0, 1, 0) \
+  X(1024, 8, 512, 2, 3, 1, 0, 1, 0) \
+  X(1024, 8, 512, 1, 3, 1, 0, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 0, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 0, 1, 1) \
   X(2048, 2, 512, 1, 3, 1, 1, 4, 0) \
