@@ -1030,6 +1030,9 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(1024, 4, 256, 1, 3, 1, 0, 1, 0) \
   X(1024, 8, 512, 2, 3, 1, 0, 1, 0) \
   X(1024, 8, 512, 1, 3, 1, 0, 1, 0) \
+  X(1024, 8, 1024, 1, 3, 1, 0, 1, 0) \
+  X(1024, 4, 512, 1, 3, 1, 0, 1, 0) \
+  X(1024, 4, 512, 2, 3, 1, 0, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 0, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 0, 1, 1) \
   X(2048, 2, 512, 1, 3, 1, 1, 4, 0) \
@@ -1140,33 +1143,37 @@ hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s) 
 // per thread), PAIRS row pairs per workgroup; the first listed for each NY is
 // the default.
 #define NS2D_RPASS_VARIANTS(X) \
-  X(8, 1, 64, 1)        \
-  X(16, 1, 64, 1)       \
-  X(32, 2, 32, 1)       \
-  X(32, 4, 16, 1)       \
-  X(64, 4, 16, 1)       \
-  X(64, 8, 8, 1)        \
-  X(128, 8, 8, 1)       \
-  X(128, 16, 4, 1)      \
-  X(128, 32, 2, 1)      \
-  X(256, 64, 1, 1)      \
-  X(256, 16, 4, 1)      \
-  X(256, 32, 2, 1)      \
-  X(512, 64, 1, 1)      \
-  X(512, 32, 2, 1)      \
-  X(512, 128, 1, 1)     \
-  X(1024, 256, 1, 1)    \
-  X(1024, 64, 1, 1)     \
-  X(1024, 128, 1, 1)    \
-  X(2048, 128, 1, 2)    \
-  X(2048, 128, 1, 1)    \
-  X(2048, 256, 1, 1)    \
-  X(2048, 512, 1, 1)    \
-  X(4096, 512, 1, 4)    \
-  X(4096, 256, 1, 1)    \
-  X(4096, 256, 1, 2)    \
-  X(4096, 512, 1, 1)    \
-  X(4096, 1024, 1, 1)  
+  X(8, 1, 64, 1) \
+  X(16, 1, 64, 1) \
+  X(32, 2, 32, 1) \
+  X(32, 4, 16, 1) \
+  X(64, 4, 16, 1) \
+  X(64, 8, 8, 1) \
+  X(128, 8, 8, 1) \
+  X(128, 16, 4, 1) \
+  X(128, 32, 2, 1) \
+  X(256, 64, 1, 1) \
+  X(256, 16, 4, 1) \
+  X(256, 32, 2, 1) \
+  X(512, 64, 1, 1) \
+  X(512, 32, 2, 1) \
+  X(512, 128, 1, 1) \
+  X(512, 64, 2, 1) \
+  X(512, 64, 4, 1) \
+  X(1024, 256, 1, 1) \
+  X(1024, 64, 1, 1) \
+  X(1024, 128, 1, 1) \
+  X(1024, 64, 2, 1) \
+  X(1024, 64, 4, 1) \
+  X(2048, 128, 1, 2) \
+  X(2048, 128, 1, 1) \
+  X(2048, 256, 1, 1) \
+  X(2048, 512, 1, 1) \
+  X(4096, 512, 1, 4) \
+  X(4096, 256, 1, 1) \
+  X(4096, 256, 1, 2) \
+  X(4096, 512, 1, 1) \
+  X(4096, 1024, 1, 1)
 
 struct RVariant {
   int NY, TLR, PAIRS, W;
