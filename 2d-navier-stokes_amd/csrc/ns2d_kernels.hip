@@ -102,9 +102,10 @@ __device__ __forceinline__ double dealias_factor(const PhysArgs& ph, int kx, int
 //              G4 slot: P = S_0, Q = S_H                  -> Re G4(x,0), Re G4(x,H)
 //            (G4(x,H) is written as two 8-byte halves from two slots; the
 //            unused parts are written as zeros.)
-// The columns the transforms need wait in pairbuf ([split][2][Nx], per
-// workgroup split): KH between the forward transform and its epilogue, then
-// the next stage inputs S_0 and S_H.
+// With PK = 1 the columns the transforms need wait in pairbuf ([split][2][Nx],
+// per workgroup split): KH between the forward transform and its epilogue,
+// then the next stage inputs S_0 and S_H.  With PK = 2 they stay in registers
+// (E more per thread), so slot 0 costs only the extra LDS exchanges.
 // ---------------------------------------------------------------------------
 // PS: LDS padding (one slot per 2^PS elements); OFF: extra 16-B slots between
 // the slots' lines, which sets where the neighbouring slot's accesses of one
@@ -136,6 +137,12 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
   double2* __restrict__ hold = a.pairbuf + (size_t)split * 2 * L;
 
   double2 R[E];  // forward result K, then the next stage input S
+  // PK = 2: the pair's second column (KH, then S_H) beside R; S_0 moves here
+  // when the second column's epilogue runs (see the column loop)
+  constexpr bool PKR = PK == 2;
+  double2 RH[PKR ? E : 1];
+#pragma unroll
+  for (int e = 0; e < (PKR ? E : 1); ++e) RH[e] = make_double2(0.0, 0.0);
 
   // PF: the RK epilogue's state, accumulator and coefficient loads are issued
   // before the forward transform, so that their latency hides behind it
@@ -187,7 +194,9 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
           const int x = l + TL * e;
           const double2 z = R[e], zm = buf[padx_s<PS>((L - x) & (L - 1))];
           R[e] = make_double2(0.5 * (z.x + zm.x), 0.5 * (z.y - zm.y));    // K0
-          hold[L + x] = make_double2(0.5 * (z.y + zm.y), -0.5 * (z.x - zm.x));  // KH
+          const double2 kh = make_double2(0.5 * (z.y + zm.y), -0.5 * (z.x - zm.x));  // KH
+          if constexpr (PKR) RH[e] = kh;
+          else hold[L + x] = kh;
         }
       }
       xsync<BLOCK>();
@@ -205,8 +214,17 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
     const int kq = opaque(q ? H : ky);
     const int lq = opaque(l);
     auto index_of = [&](int it) -> size_t { return (size_t)(lq + TL * it) * ld + kq; };
+    if (PKR && q == 1) {
+      // S_0 to RH, KH (forward modes) to R
+#pragma unroll
+      for (int e = 0; e < (PKR ? E : 1); ++e) {
+        const double2 t = R[e];
+        R[e] = RH[e];
+        RH[e] = t;
+      }
+    }
     if (a.mode & KP_FWD) {
-      if (q == 1) {
+      if (!PKR && q == 1) {
 #pragma unroll
         for (int e = 0; e < E; ++e) R[e] = hold[L + lq + TL * e];
       }
@@ -347,7 +365,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
 #pragma unroll
       for (int it = 0; it < E; ++it) a.conj_dst[lq + TL * it] = R[it];
     }
-    if (pair && (a.mode & KP_INV)) {
+    if (!PKR && pair && (a.mode & KP_INV)) {
 #pragma unroll
       for (int it = 0; it < E; ++it) hold[q * L + lq + TL * it] = R[it];
     }
@@ -391,7 +409,9 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
         for (int e = 0; e < E; ++e) {
           const int x = l_o + TL * e;
           const int kx = kx_of_row(x, nx);
-          const double2 s0 = hold[x], sh = hold[L + x];
+          // S_0 and S_H: after the column loop RH holds S_0 and R holds S_H (PK = 2)
+          const double2 s0 = PKR ? RH[PKR ? e : 0] : hold[x];
+          const double2 sh = PKR ? R[e] : hold[L + x];
           double2 P, Q;
           if (f == 0) {
             const double2 ps = derive(0, sh, kx, H);
@@ -971,9 +991,9 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
 // kpass variants {L, C, T, KS, PS, OFF, PK, W, PF}: L-point columns, C column
 // slots per workgroup, T threads, KS workgroups per column group, LDS padding
 // shift PS and line offset OFF, PK = 1 packs the ky = 0 and ky = Ny/2 columns
-// into slot 0, W minimum waves per SIMD, PF = 1 issues the RK epilogue's
-// loads before the forward transform; the first listed for each L is the
-// default.  Rows per thread E = L*C/T.  Measured on square grids: up to
+// into slot 0 (PK = 2: with the pair's second column in registers), W minimum
+// waves per SIMD, PF = 1 issues the RK epilogue's loads before the forward
+// transform; the first listed for each L is the default.  Rows per thread E = L*C/T.  Measured on square grids: up to
 // L = 2048 the unpacked shape is faster (the packed pair's extra exchanges and
 // held columns make slot 0 the last workgroup to finish, and at 2048 its
 // register budget costs occupancy); at 4096, one workgroup per CU, the packed
@@ -1010,6 +1030,7 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(512, 8, 512, 2, 3, 1, 0, 1, 0) \
   X(512, 8, 512, 1, 3, 1, 0, 1, 0) \
   X(512, 16, 512, 2, 3, 1, 0, 1, 0) \
+  X(512, 4, 256, 2, 3, 1, 2, 1, 0) \
   X(1024, 2, 256, 2, 3, 1, 0, 1, 0) \
   X(1024, 2, 256, 2, 3, 1, 0, 1, 1) \
   X(1024, 2, 256, 2, 3, 1, 1, 1, 0) \
@@ -1033,6 +1054,8 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(1024, 8, 1024, 1, 3, 1, 0, 1, 0) \
   X(1024, 4, 512, 1, 3, 1, 0, 1, 0) \
   X(1024, 4, 512, 2, 3, 1, 0, 1, 0) \
+  X(1024, 2, 256, 2, 3, 1, 2, 1, 0) \
+  X(1024, 2, 256, 2, 3, 1, 2, 3, 0) \
   X(2048, 2, 512, 1, 3, 1, 0, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 0, 1, 1) \
   X(2048, 2, 512, 1, 3, 1, 1, 4, 0) \
@@ -1041,6 +1064,8 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(2048, 4, 512, 1, 3, 1, 1, 1, 0) \
   X(2048, 1, 128, 4, 3, 1, 1, 1, 0) \
   X(2048, 4, 1024, 1, 3, 1, 1, 1, 0) \
+  X(2048, 2, 512, 1, 3, 1, 2, 1, 0) \
+  X(2048, 2, 512, 1, 3, 1, 2, 4, 0) \
   X(4096, 2, 512, 1, 3, 1, 1, 1, 0) \
   X(4096, 2, 512, 1, 4, 4, 1, 1, 0) \
   X(4096, 2, 512, 1, 4, 8, 1, 1, 0) \
@@ -1110,7 +1135,7 @@ int kpass_variant_extra(int nx, int idx, int* W, int* PS, int* OFF, int* PK) {
   *W = v->W;
   *PS = v->PS;
   *OFF = v->OFF;
-  *PK = v->PK | (v->PF << 1);
+  *PK = (v->PK != 0) | (v->PF << 1) | ((v->PK == 2) << 2);
   return 0;
 }
 

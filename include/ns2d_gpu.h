@@ -172,8 +172,9 @@ int ns2d_variant_info(int pass, int n, int idx, int* shape4);
  * per 2^shift elements), 16-B slots between neighbouring lines, flags}; flag
  * bit 0: the column pass carries the ky = 0 and ky = Ny/2 columns as one
  * packed complex line (Ny/2 column slots instead of Ny/2+1); bit 1: the
- * column pass issues its RK epilogue loads before the forward transform
- * (flags are 0 for the row pass). */
+ * column pass issues its RK epilogue loads before the forward transform;
+ * bit 2: the packed pair's second column stays in registers (flags are 0 for
+ * the row pass). */
 int ns2d_variant_extra(int pass, int n, int idx, int* extra4);
 int ns2d_set_variant(ns2d_t* h, int pass, int idx);
 

@@ -181,6 +181,10 @@ def _pair_cases():
         packed = [i for i, v in enumerate(vs) if v["packed_pair"]]
         unpacked = [i for i, v in enumerate(vs) if not v["packed_pair"]]
         cases.append((nx, ny, packed[0], unpacked[0]))
+        # the shape that keeps the pair's second column in registers
+        regs = [i for i, v in enumerate(vs) if v["pair_in_registers"]]
+        if regs:
+            cases.append((nx, ny, regs[0], unpacked[0]))
     return cases
 
 
