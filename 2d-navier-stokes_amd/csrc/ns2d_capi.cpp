@@ -1072,9 +1072,11 @@ int ens_autotune(ns2d_ensemble* e, int reps) {
     rc = read_stat(e->m[i], &saved[i * STAT_N]);
     if (rc) return rc;
   }
-  hipEvent_t t0, t1;
-  HIPCHK(hipEventCreate(&t0));
-  HIPCHK(hipEventCreate(&t1));
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess) {
+    if (t0) (void)hipEventDestroy(t0);
+    return fail(NS2D_ERR_HIP, "event create failed");
+  }
   auto time_it = [&](auto&& launch) -> float {
     float best = 1e30f;
     for (int r = 0; r < reps; ++r) {
