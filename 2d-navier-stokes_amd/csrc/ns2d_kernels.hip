@@ -492,7 +492,11 @@ __global__ __launch_bounds__(T, W) void kpass_batched(const KPassArgs* __restric
 // and transformed forward once; N(x0,k) = (W[k] + conj W[NY-k])/2 and
 // N(x0+1,k) = (W[k] - conj W[NY-k])/(2i).
 // ---------------------------------------------------------------------------
-template <int NY, int TLR, int PAIRS>
+// PFR: the four G fields of both rows are loaded at the start (2 x 4 x E
+// values per thread, for shapes with few points per thread), so that the
+// loads of the second row and of each row's second field pair overlap the
+// first transforms
+template <int NY, int TLR, int PAIRS, int PFR>
 __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   constexpr int T = TLR * PAIRS;
   constexpr int H = NY / 2;
@@ -541,7 +545,55 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
 
   double n0[E];
   double2 w[E];
-  if ((a.mode & RP_NL) && (BLOCK || valid)) {
+  if constexpr (PFR != 0) {
+    if ((a.mode & RP_NL) && (BLOCK || valid)) {
+      double2 pre[2][4][E];
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            const int ky = l + TLR * e;
+            const int kk = ky > H ? NY - ky : ky;
+            pre[r][f][e] = a.g[f][(size_t)(x0 + r) * ld + kk];
+          }
+      // the same assembly as build() from the loaded values
+      auto build_pre = [&](double2* z, int which, const double2 (&g)[4][E]) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int ky = l + TLR * e;
+          const bool mirror = ky > H;
+          const int kk = mirror ? NY - ky : ky;
+          const double fk = (double)kk;
+          double2 A, B;
+          if (which == 0) {
+            A = make_double2(-fk * g[0][e].y, fk * g[0][e].x);
+            B = g[1][e];
+          } else {
+            A = g[2][e];
+            B = make_double2(-fk * g[3][e].y, fk * g[3][e].x);
+          }
+          if (kk == 0 || kk == H) { A.y = 0.0; B.y = 0.0; }
+          z[e] = mirror ? make_double2(A.x + B.y, B.x - A.y) : make_double2(A.x - B.y, A.y + B.x);
+        }
+      };
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        double2 z1[E], z2[E];
+        build_pre(z1, 0, pre[r]);
+        fft_reg<NY, TLR, +1, BLOCK>(z1, buf, tw, opaque(l));
+        build_pre(z2, 1, pre[r]);
+        fft_reg<NY, TLR, +1, BLOCK>(z2, buf, tw, opaque(l));
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const double nl = z1[e].x * z2[e].x + z1[e].y * z2[e].y;   // u*w_x + v*w_y
+          if (r == 0) n0[e] = nl;
+          else w[e] = make_double2(n0[e], nl);
+        }
+      }
+    }
+  } else if ((a.mode & RP_NL) && (BLOCK || valid)) {
 #pragma unroll 1
     for (int r = 0; r < 2; ++r) {
       const int x = opaque(x0 + r);
@@ -619,15 +671,15 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
 }
 
 // W: minimum waves per SIMD the register allocation must allow (1 = no limit)
-template <int NY, int TLR, int PAIRS, int W>
+template <int NY, int TLR, int PAIRS, int W, int PFR>
 __global__ __launch_bounds__(TLR * PAIRS, W) void rpass_kernel(RPassArgs a) {
-  rpass_body<NY, TLR, PAIRS>(a, (int)blockIdx.x);
+  rpass_body<NY, TLR, PAIRS, PFR>(a, (int)blockIdx.x);
 }
 
-template <int NY, int TLR, int PAIRS, int W>
+template <int NY, int TLR, int PAIRS, int W, int PFR>
 __global__ __launch_bounds__(TLR * PAIRS, W) void rpass_batched(const RPassArgs* __restrict__ tab, int per_member) {
   const int m = (int)blockIdx.x / per_member;
-  rpass_body<NY, TLR, PAIRS>(tab[m], (int)blockIdx.x - m * per_member);
+  rpass_body<NY, TLR, PAIRS, PFR>(tab[m], (int)blockIdx.x - m * per_member);
 }
 
 // ---------------------------------------------------------------------------
@@ -1164,50 +1216,56 @@ hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s) 
   return hipGetLastError();
 }
 
-// rpass variants {NY, TLR, PAIRS}: TLR threads per row pair (E = NY/TLR points
-// per thread), PAIRS row pairs per workgroup; the first listed for each NY is
-// the default.
+// rpass variants {NY, TLR, PAIRS, W, PFR}: TLR threads per row pair (E = NY/TLR
+// points per thread), PAIRS row pairs per workgroup, W minimum waves per SIMD,
+// PFR = 1 loads both rows' G fields up front (shapes with E = 4); the first
+// listed for each NY is the default.
 #define NS2D_RPASS_VARIANTS(X) \
-  X(8, 1, 64, 1) \
-  X(16, 1, 64, 1) \
-  X(32, 2, 32, 1) \
-  X(32, 4, 16, 1) \
-  X(64, 4, 16, 1) \
-  X(64, 8, 8, 1) \
-  X(128, 8, 8, 1) \
-  X(128, 16, 4, 1) \
-  X(128, 32, 2, 1) \
-  X(256, 64, 1, 1) \
-  X(256, 16, 4, 1) \
-  X(256, 32, 2, 1) \
-  X(512, 64, 1, 1) \
-  X(512, 32, 2, 1) \
-  X(512, 128, 1, 1) \
-  X(512, 64, 2, 1) \
-  X(512, 64, 4, 1) \
-  X(1024, 256, 1, 1) \
-  X(1024, 64, 1, 1) \
-  X(1024, 128, 1, 1) \
-  X(1024, 64, 2, 1) \
-  X(1024, 64, 4, 1) \
-  X(2048, 128, 1, 2) \
-  X(2048, 128, 1, 1) \
-  X(2048, 256, 1, 1) \
-  X(2048, 512, 1, 1) \
-  X(4096, 512, 1, 4) \
-  X(4096, 256, 1, 1) \
-  X(4096, 256, 1, 2) \
-  X(4096, 512, 1, 1) \
-  X(4096, 1024, 1, 1)
+  X(8, 1, 64, 1, 0) \
+  X(16, 1, 64, 1, 0) \
+  X(32, 2, 32, 1, 0) \
+  X(32, 4, 16, 1, 0) \
+  X(64, 4, 16, 1, 0) \
+  X(64, 8, 8, 1, 0) \
+  X(128, 8, 8, 1, 0) \
+  X(128, 16, 4, 1, 0) \
+  X(128, 32, 2, 1, 0) \
+  X(256, 64, 1, 1, 0) \
+  X(256, 64, 1, 1, 1) \
+  X(256, 16, 4, 1, 0) \
+  X(256, 32, 2, 1, 0) \
+  X(512, 64, 1, 1, 0) \
+  X(512, 32, 2, 1, 0) \
+  X(512, 128, 1, 1, 0) \
+  X(512, 128, 1, 1, 1) \
+  X(512, 64, 2, 1, 0) \
+  X(512, 64, 4, 1, 0) \
+  X(1024, 256, 1, 1, 0) \
+  X(1024, 256, 1, 1, 1) \
+  X(1024, 64, 1, 1, 0) \
+  X(1024, 128, 1, 1, 0) \
+  X(1024, 64, 2, 1, 0) \
+  X(1024, 64, 4, 1, 0) \
+  X(2048, 128, 1, 2, 0) \
+  X(2048, 128, 1, 1, 0) \
+  X(2048, 256, 1, 1, 0) \
+  X(2048, 512, 1, 1, 0) \
+  X(2048, 512, 1, 1, 1) \
+  X(4096, 512, 1, 4, 0) \
+  X(4096, 256, 1, 1, 0) \
+  X(4096, 256, 1, 2, 0) \
+  X(4096, 512, 1, 1, 0) \
+  X(4096, 1024, 1, 1, 0) \
+  X(4096, 1024, 1, 1, 1)
 
 struct RVariant {
-  int NY, TLR, PAIRS, W;
+  int NY, TLR, PAIRS, W, PFR;
   void (*kern)(RPassArgs);
   void (*batched)(const RPassArgs*, int);
 };
 
-#define NS2D_RV_ENTRY(NY_, TLR_, PAIRS_, W_) \
-  {NY_, TLR_, PAIRS_, W_, rpass_kernel<NY_, TLR_, PAIRS_, W_>, rpass_batched<NY_, TLR_, PAIRS_, W_>},
+#define NS2D_RV_ENTRY(NY_, TLR_, PAIRS_, W_, PFR_) \
+  {NY_, TLR_, PAIRS_, W_, PFR_, rpass_kernel<NY_, TLR_, PAIRS_, W_, PFR_>, rpass_batched<NY_, TLR_, PAIRS_, W_, PFR_>},
 static const RVariant rvariants[] = {NS2D_RPASS_VARIANTS(NS2D_RV_ENTRY)};
 #undef NS2D_RV_ENTRY
 
@@ -1250,7 +1308,7 @@ int rpass_variant_extra(int ny, int idx, int* W, int* PS, int* OFF, int* PK) {
   *W = v->W;
   *PS = 3;
   *OFF = 1;
-  *PK = 0;
+  *PK = v->PFR << 1;
   return 0;
 }
 
