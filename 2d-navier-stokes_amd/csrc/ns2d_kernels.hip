@@ -1045,11 +1045,13 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
 // shift PS and line offset OFF, PK = 1 packs the ky = 0 and ky = Ny/2 columns
 // into slot 0 (PK = 2: with the pair's second column in registers), W minimum
 // waves per SIMD, PF = 1 issues the RK epilogue's loads before the forward
-// transform; the first listed for each L is the default.  Rows per thread E = L*C/T.  Measured on square grids: up to
-// L = 2048 the unpacked shape is faster (the packed pair's extra exchanges and
-// held columns make slot 0 the last workgroup to finish, and at 2048 its
-// register budget costs occupancy); at 4096, one workgroup per CU, the packed
-// pair removes the odd column group of the last round (-9 %).
+// transform; the first listed for each L is the default.  Rows per thread
+// E = L*C/T.  Measured on square grids: up to L = 2048 the unpacked shape is
+// faster (the packed pair's extra exchanges and held columns make slot 0 the
+// last workgroup to finish, and at 2048 its register budget costs
+// occupancy); at 4096, one workgroup per CU, the packed pair removes the odd
+// column group of the last round (-9 %), and 1024 threads with 8 rows each
+// and a 4-slot line offset beat 512 threads with 16 rows (-5 %).
 #define NS2D_KPASS_VARIANTS(X) \
   X(8, 16, 64, 1, 3, 1, 0, 1, 0) \
   X(8, 16, 64, 1, 3, 1, 1, 1, 0) \
@@ -1118,6 +1120,7 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(2048, 4, 1024, 1, 3, 1, 1, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 2, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 2, 4, 0) \
+  X(4096, 2, 1024, 1, 3, 4, 1, 1, 0) \
   X(4096, 2, 512, 1, 3, 1, 1, 1, 0) \
   X(4096, 2, 512, 1, 4, 4, 1, 1, 0) \
   X(4096, 2, 512, 1, 4, 8, 1, 1, 0) \
@@ -1126,7 +1129,6 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(4096, 1, 256, 2, 3, 1, 1, 1, 0) \
   X(4096, 2, 512, 2, 3, 1, 1, 1, 0) \
   X(4096, 2, 1024, 1, 3, 1, 1, 1, 0) \
-  X(4096, 2, 1024, 1, 3, 4, 1, 1, 0) \
   X(4096, 2, 1024, 1, 3, 8, 1, 1, 0) \
   X(4096, 1, 512, 1, 3, 1, 1, 1, 0) \
   X(4096, 1, 512, 2, 3, 1, 1, 1, 0) \
