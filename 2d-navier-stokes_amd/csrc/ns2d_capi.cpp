@@ -387,6 +387,23 @@ int ns2d_variant_extra(int pass, int n, int idx, int* extra4) {
   return NS2D_OK;
 }
 
+int ns2d_default_variant(int pass, int n, int integrator, int* idx) {
+  if (!idx) return fail(NS2D_ERR_INVALID, "null argument");
+  if (integrator != NS2D_INT_RK4CN && integrator != NS2D_INT_AB4)
+    return fail(NS2D_ERR_INVALID, "unknown integrator %d", integrator);
+  if (pass == NS2D_PASS_COLUMN) {
+    if (kpass_variant_count(n) == 0) return fail(NS2D_ERR_INVALID, "no column-pass shape for Nx=%d", n);
+    *idx = kpass_default_variant(n, integrator == NS2D_INT_AB4);
+    return NS2D_OK;
+  }
+  if (pass == NS2D_PASS_ROW) {
+    if (rpass_variant_count(n) == 0) return fail(NS2D_ERR_INVALID, "no row-pass shape for Ny=%d", n);
+    *idx = 0;
+    return NS2D_OK;
+  }
+  return fail(NS2D_ERR_INVALID, "unknown pass %d", pass);
+}
+
 int ns2d_set_variant(ns2d_t* h, int pass, int idx) {
   if (!h) return fail(NS2D_ERR_INVALID, "null handle");
   if (pass == NS2D_PASS_COLUMN) {
@@ -535,6 +552,7 @@ static int create_impl(const ns2d_params* p, hipStream_t ext_stream, ns2d_t** ou
   if (const char* g = std::getenv("NS2D_GRAPH")) h->use_graph = std::atoi(g) != 0;
   // pass shapes: built-in defaults, or NS2D_KPASS_VARIANT_<Nx> / NS2D_RPASS_VARIANT_<Ny>
   {
+    h->kvar = kpass_default_variant(p->nx, p->integrator == NS2D_INT_AB4);
     char name[64];
     std::snprintf(name, sizeof name, "NS2D_KPASS_VARIANT_%d", p->nx);
     const char* env = std::getenv(name);

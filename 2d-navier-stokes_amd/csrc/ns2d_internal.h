@@ -133,6 +133,7 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
 
 hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s);
 int kpass_variant_count(int nx);
+int kpass_default_variant(int nx, bool ab4);
 hipError_t launch_kpass_batched(int nx, int nyf, int variant, const KPassArgs* tab, int members, hipStream_t s);
 hipError_t launch_rpass_batched(int nx, int ny, int variant, const RPassArgs* tab, int members, hipStream_t s);
 int kpass_variant_info(int nx, int idx, int* C, int* E, int* T, int* KS);

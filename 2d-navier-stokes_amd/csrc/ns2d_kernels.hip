@@ -1167,6 +1167,24 @@ hipError_t launch_kpass_batched(int nx, int nyf, int variant, const KPassArgs* t
   return hipGetLastError();
 }
 
+// Default column-pass shape: index 0 of its length, except CN-AB4 at 4096.
+// Its epilogue reads three history fields besides the state, and with them
+// 512 threads of 16 rows each beat the 1024-thread shape (600 vs 618 us per
+// launch on MI355X), which the lighter RK4CN stage epilogues prefer
+// (437 vs 451 us).
+int kpass_default_variant(int nx, bool ab4) {
+  if (ab4 && nx == 4096) {
+    int n = 0;
+    for (const KVariant& v : kvariants) {
+      if (v.L != nx) continue;
+      if (v.C == 2 && v.T == 512 && v.KS == 1 && v.PS == 3 && v.OFF == 1 && v.PK == 1 && v.W == 1 && v.PF == 0)
+        return n;
+      ++n;
+    }
+  }
+  return 0;
+}
+
 int kpass_variant_count(int nx) {
   int n = 0;
   for (const KVariant& v : kvariants) n += v.L == nx;

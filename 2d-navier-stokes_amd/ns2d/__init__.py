@@ -64,7 +64,7 @@ EXPORTS = ["ns2d_params_default", "ns2d_create", "ns2d_destroy", "ns2d_set_state
            "ns2d_set_state_device", "ns2d_get_state_device", "ns2d_step", "ns2d_rhs", "ns2d_measures",
            "ns2d_max_vort", "ns2d_real_vort", "ns2d_r2c", "ns2d_c2r", "ns2d_get_info", "ns2d_sync",
            "ns2d_get_stream", "ns2d_profile_steps", "ns2d_variant_count", "ns2d_variant_info", "ns2d_variant_extra",
-           "ns2d_set_variant", "ns2d_ens_create", "ns2d_ens_member", "ns2d_ens_step", "ns2d_ens_autotune", "ns2d_ens_variants", "ns2d_ens_sync",
+           "ns2d_default_variant", "ns2d_set_variant", "ns2d_ens_create", "ns2d_ens_member", "ns2d_ens_step", "ns2d_ens_autotune", "ns2d_ens_variants", "ns2d_ens_sync",
            "ns2d_ens_destroy", "ns2d_vjp", "ns2d_vjp_device", "ns2d_last_error"]
 
 _lib = None
@@ -118,6 +118,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ns2d_variant_count": ([i, i], i),
         "ns2d_variant_info": ([i, i, i, P(i)], i),
         "ns2d_variant_extra": ([i, i, i, P(i)], i),
+        "ns2d_default_variant": ([i, i, i, P(i)], i),
         "ns2d_set_variant": ([vp, i, i], i),
         "ns2d_ens_create": ([P(_Params), i, P(vp)], i),
         "ns2d_ens_member": ([vp, i, P(vp)], i),
@@ -167,6 +168,15 @@ def kpass_variants(nx: int) -> list:
 
 def rpass_variants(ny: int) -> list:
     return variants(PASS_ROW, ny)
+
+
+def default_variant(pass_kind: int, n: int, integrator: str = "RK4CN") -> int:
+    """Index of the shape a new solver with this integrator runs for an
+    n-point transform (before NS2D_KPASS_VARIANT_<Nx> / NS2D_RPASS_VARIANT_<Ny>)."""
+    lib = load_library()
+    idx = ctypes.c_int()
+    _check(lib.ns2d_default_variant(pass_kind, n, INTEGRATORS[integrator], ctypes.byref(idx)))
+    return idx.value
 
 
 class NS2DError(RuntimeError):

@@ -160,9 +160,10 @@ int ns2d_profile_steps(ns2d_t* h, double dt, long nsteps, double* kpass_ms, doub
  *                      workgroups sharing a column group}
  *   row pass (Ny):    {threads per row pair, points per thread, threads,
  *                      row pairs per workgroup}
- * The project's autotuner times them and picks one per grid; index 0 is the
- * built-in default.  No counterpart in the reference (FFTW_MEASURE planning,
- * solver.c:2883-2895, plays this role there). */
+ * The project's autotuner times them and picks one per grid; a new handle
+ * starts from ns2d_default_variant's shape (index 0, except the column pass
+ * of CN-AB4 at Nx = 4096).  No counterpart in the reference (FFTW_MEASURE
+ * planning, solver.c:2883-2895, plays this role there). */
 #define NS2D_PASS_COLUMN 0
 #define NS2D_PASS_ROW 1
 int ns2d_variant_count(int pass, int n);
@@ -176,6 +177,10 @@ int ns2d_variant_info(int pass, int n, int idx, int* shape4);
  * bit 2: the packed pair's second column stays in registers (flags are 0 for
  * the row pass). */
 int ns2d_variant_extra(int pass, int n, int idx, int* extra4);
+/* The shape a new handle with this integrator (NS2D_INT_*) runs for a
+ * transform length n unless NS2D_KPASS_VARIANT_<Nx> / NS2D_RPASS_VARIANT_<Ny>
+ * name another.  Needs no GPU. */
+int ns2d_default_variant(int pass, int n, int integrator, int* idx);
 int ns2d_set_variant(ns2d_t* h, int pass, int idx);
 
 /* Ensemble (no counterpart in the reference, which runs one realisation per

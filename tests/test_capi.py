@@ -37,6 +37,16 @@ def test_variant_tables_cover_every_size():
         assert not any(v["packed_pair"] and v["prefetch"] for v in vs), n
         assert vs[0]["packed_pair"] == (1 if n == 4096 else 0), n
         assert all(v["min_waves"] >= 1 and v["pad_shift"] >= 3 for v in vs), n
+        # defaults: index 0 of each length, except CN-AB4's column pass at
+        # 4096 (a packed 512-thread shape; RK4CN's is the 1024-thread one)
+        assert ns2d.default_variant(ns2d.PASS_ROW, n, "AB4") == 0
+        assert ns2d.default_variant(ns2d.PASS_COLUMN, n, "RK4CN") == 0
+        ab4 = ns2d.default_variant(ns2d.PASS_COLUMN, n, "AB4")
+        if n == 4096:
+            assert ab4 > 0 and vs[ab4]["threads"] == 512 and vs[ab4]["cols"] == 2 and vs[ab4]["packed_pair"]
+            assert vs[ab4]["splits"] == 1 and not vs[ab4]["prefetch"] and vs[0]["threads"] == 1024
+        else:
+            assert ab4 == 0, n
         rs = ns2d.rpass_variants(n)
         assert rs, n
         for v in rs:
