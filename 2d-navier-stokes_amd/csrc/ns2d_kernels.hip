@@ -1051,7 +1051,9 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
 // last workgroup to finish, and at 2048 its register budget costs
 // occupancy); at 4096, one workgroup per CU, the packed pair removes the odd
 // column group of the last round (-9 %), and 1024 threads with 8 rows each
-// and a 4-slot line offset beat 512 threads with 16 rows (-5 %).
+// and a 4-slot line offset beat 512 threads with 16 rows (-3 %; CN-AB4 keeps
+// the latter, kpass_default_variant).  At 1024, 512 threads with 4 rows each
+// beat 256 threads with 8 rows (-2 %, shorter per-workgroup chains).
 #define NS2D_KPASS_VARIANTS(X) \
   X(8, 16, 64, 1, 3, 1, 0, 1, 0) \
   X(8, 16, 64, 1, 3, 1, 1, 1, 0) \
@@ -1085,6 +1087,7 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(512, 8, 512, 1, 3, 1, 0, 1, 0) \
   X(512, 16, 512, 2, 3, 1, 0, 1, 0) \
   X(512, 4, 256, 2, 3, 1, 2, 1, 0) \
+  X(1024, 2, 512, 2, 3, 1, 0, 1, 0) \
   X(1024, 2, 256, 2, 3, 1, 0, 1, 0) \
   X(1024, 2, 256, 2, 3, 1, 0, 1, 1) \
   X(1024, 2, 256, 2, 3, 1, 1, 1, 0) \
@@ -1110,6 +1113,8 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(1024, 4, 512, 2, 3, 1, 0, 1, 0) \
   X(1024, 2, 256, 2, 3, 1, 2, 1, 0) \
   X(1024, 2, 256, 2, 3, 1, 2, 3, 0) \
+  X(1024, 2, 256, 4, 3, 1, 0, 1, 0) \
+  X(1024, 2, 512, 4, 3, 1, 0, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 0, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 0, 1, 1) \
   X(2048, 2, 512, 1, 3, 1, 1, 4, 0) \
