@@ -492,10 +492,15 @@ __global__ __launch_bounds__(T, W) void kpass_batched(const KPassArgs* __restric
 // and transformed forward once; N(x0,k) = (W[k] + conj W[NY-k])/2 and
 // N(x0+1,k) = (W[k] - conj W[NY-k])/(2i).
 // ---------------------------------------------------------------------------
-// PFR: the four G fields of both rows are loaded at the start (2 x 4 x E
+// PFR = 1: the four G fields of both rows are loaded at the start (2 x 4 x E
 // values per thread, for shapes with few points per thread), so that the
 // loads of the second row and of each row's second field pair overlap the
-// first transforms
+// first transforms.  PFR = 2 (single row): TLR threads own one row x0 = blk
+// instead of a pair; the product n(x0) is transformed forward on its own
+// (imaginary part zero) and N(x0,k) = W[k] with the imaginary parts of k = 0
+// and NY/2 set to zero, as a real-input transform gives them.  Twice the
+// workgroups and three transforms per workgroup instead of five, for one
+// more forward transform per row pair.
 template <int NY, int TLR, int PAIRS, int PFR>
 __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   constexpr int T = TLR * PAIRS;
@@ -503,20 +508,22 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   constexpr int E = NY / TLR;
   constexpr int LS = line_stride(NY);
   constexpr bool BLOCK = TLR > 64;
+  constexpr bool SR = PFR == 2;
   static_assert(E >= 2 && (BLOCK ? PAIRS == 1 : 64 % TLR == 0), "bad rpass shape");
+  static_assert(!SR || PAIRS == 1, "single-row shapes own one row per workgroup");
   __shared__ double2 lds[PAIRS * LS];
   __shared__ double red[(T + 63) / 64];
 
   const int tid = threadIdx.x;
   const int q = tid / TLR, l = tid % TLR;
-  const int x0 = 2 * (blk * PAIRS + q);
+  const int x0 = SR ? blk : 2 * (blk * PAIRS + q);
   const bool valid = x0 < a.nx;   // uniform per block when BLOCK
   const int ld = a.ld;
   const double2* __restrict__ tw = a.tw;
   double2* buf = lds + q * LS;
 
   // full-length spectrum of A + iB for row x (which: 0 -> u + iv, 1 -> w_x + i w_y,
-  // 2 -> w(x) + i w(x+1))
+  // 2 -> w(x) + i w(x+1), 3 -> w(x))
   auto build = [&](double2* z, int which, int x, int lo) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -533,9 +540,12 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
         A = a.g[2][(size_t)x * ld + kk];
         const double2 g4 = a.g[3][(size_t)x * ld + kk];
         B = make_double2(-fk * g4.y, fk * g4.x);
-      } else {                  // w of rows x and x+1 = G4
+      } else if (which == 2) {  // w of rows x and x+1 = G4
         A = a.g[3][(size_t)x * ld + kk];
         B = a.g[3][(size_t)(x + 1) * ld + kk];
+      } else {                  // w of row x alone
+        A = a.g[3][(size_t)x * ld + kk];
+        B = make_double2(0.0, 0.0);
       }
       if (kk == 0 || kk == H) { A.y = 0.0; B.y = 0.0; }
       z[e] = mirror ? make_double2(A.x + B.y, B.x - A.y) : make_double2(A.x - B.y, A.y + B.x);
@@ -545,7 +555,7 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
 
   double n0[E];
   double2 w[E];
-  if constexpr (PFR != 0) {
+  if constexpr (PFR == 1) {
     if ((a.mode & RP_NL) && (BLOCK || valid)) {
       double2 pre[2][4][E];
 #pragma unroll
@@ -593,6 +603,14 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
         }
       }
     }
+  } else if (SR && (a.mode & RP_NL) && (BLOCK || valid)) {
+    double2 z1[E], z2[E];
+    build(z1, 0, x0, opaque(l));
+    fft_reg<NY, TLR, +1, BLOCK>(z1, buf, tw, opaque(l));
+    build(z2, 1, x0, opaque(l));
+    fft_reg<NY, TLR, +1, BLOCK>(z2, buf, tw, opaque(l));
+#pragma unroll
+    for (int e = 0; e < E; ++e) w[e] = make_double2(z1[e].x * z2[e].x + z1[e].y * z2[e].y, 0.0);
   } else if ((a.mode & RP_NL) && (BLOCK || valid)) {
 #pragma unroll 1
     for (int r = 0; r < 2; ++r) {
@@ -615,15 +633,15 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
     double m = 0.0;
     if (BLOCK || valid) {
       double2 z[E];
-      build(z, 2, x0, opaque(l));
+      build(z, SR ? 3 : 2, x0, opaque(l));
       fft_reg<NY, TLR, +1, BLOCK>(z, buf, tw, opaque(l));
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int y = l + TLR * e;
-        const double w0 = z[e].x * a.wscale, w1 = z[e].y * a.wscale;
+        const double w0 = z[e].x * a.wscale, w1 = SR ? 0.0 : z[e].y * a.wscale;
         if (a.wreal) {
           a.wreal[(size_t)x0 * NY + y] = w0;
-          a.wreal[(size_t)(x0 + 1) * NY + y] = w1;
+          if (!SR) a.wreal[(size_t)(x0 + 1) * NY + y] = w1;
         }
         const double aw = fmax(fabs(w0), fabs(w1));
         m = (aw > m || aw != aw) ? aw : m;
@@ -649,7 +667,15 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
     }
   }
 
-  if ((a.mode & RP_NL) && (BLOCK || valid)) {
+  if (SR && (a.mode & RP_NL) && (BLOCK || valid)) {
+    fft_reg<NY, TLR, -1, BLOCK>(w, buf, tw, opaque(l));
+    double2* __restrict__ out0 = a.nout + (size_t)x0 * ld;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = l + TLR * e;
+      if (k <= H) out0[k] = (k == 0 || k == H) ? make_double2(w[e].x, 0.0) : w[e];
+    }
+  } else if ((a.mode & RP_NL) && (BLOCK || valid)) {
     fft_reg<NY, TLR, -1, BLOCK>(w, buf, tw, opaque(l));
     // W[NY-k] for the thread's own k: exchange through the line buffer
 #pragma unroll
@@ -1243,8 +1269,11 @@ hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s) 
 
 // rpass variants {NY, TLR, PAIRS, W, PFR}: TLR threads per row pair (E = NY/TLR
 // points per thread), PAIRS row pairs per workgroup, W minimum waves per SIMD,
-// PFR = 1 loads both rows' G fields up front (shapes with E = 4); the first
-// listed for each NY is the default.
+// PFR = 1 loads both rows' G fields up front (shapes with E = 4), PFR = 2 runs
+// one row per TLR threads instead of a pair; the first listed for each NY is
+// the default.  Single rows win at 1024 (20.5 against 21.8 us per launch: the
+// one-round grid is bound by a workgroup's chain of transforms) and lose at
+// 4096 (224 against 205 us: four rounds, where the extra transform counts).
 #define NS2D_RPASS_VARIANTS(X) \
   X(8, 1, 64, 1, 0) \
   X(16, 1, 64, 1, 0) \
@@ -1259,29 +1288,37 @@ hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s) 
   X(256, 64, 1, 1, 1) \
   X(256, 16, 4, 1, 0) \
   X(256, 32, 2, 1, 0) \
+  X(256, 64, 1, 1, 2) \
   X(512, 64, 1, 1, 0) \
   X(512, 32, 2, 1, 0) \
   X(512, 128, 1, 1, 0) \
   X(512, 128, 1, 1, 1) \
   X(512, 64, 2, 1, 0) \
   X(512, 64, 4, 1, 0) \
+  X(512, 64, 1, 1, 2) \
+  X(512, 128, 1, 1, 2) \
+  X(1024, 256, 1, 1, 2) \
   X(1024, 256, 1, 1, 0) \
   X(1024, 256, 1, 1, 1) \
   X(1024, 64, 1, 1, 0) \
   X(1024, 128, 1, 1, 0) \
   X(1024, 64, 2, 1, 0) \
   X(1024, 64, 4, 1, 0) \
+  X(1024, 128, 1, 1, 2) \
   X(2048, 128, 1, 2, 0) \
   X(2048, 128, 1, 1, 0) \
   X(2048, 256, 1, 1, 0) \
   X(2048, 512, 1, 1, 0) \
   X(2048, 512, 1, 1, 1) \
+  X(2048, 512, 1, 1, 2) \
   X(4096, 512, 1, 4, 0) \
   X(4096, 256, 1, 1, 0) \
   X(4096, 256, 1, 2, 0) \
   X(4096, 512, 1, 1, 0) \
   X(4096, 1024, 1, 1, 0) \
-  X(4096, 1024, 1, 1, 1)
+  X(4096, 1024, 1, 1, 1) \
+  X(4096, 512, 1, 4, 2) \
+  X(4096, 1024, 1, 1, 2)
 
 struct RVariant {
   int NY, TLR, PAIRS, W, PFR;
@@ -1306,7 +1343,8 @@ static const RVariant* find_rvariant(int ny, int idx) {
 hipError_t launch_rpass_batched(int nx, int ny, int variant, const RPassArgs* tab, int members, hipStream_t s) {
   const RVariant* v = find_rvariant(ny, variant);
   if (!v) return hipErrorInvalidValue;
-  const int per = (nx / 2 + v->PAIRS - 1) / v->PAIRS;
+  const int units = v->PFR == 2 ? nx : nx / 2;   // rows or row pairs
+  const int per = (units + v->PAIRS - 1) / v->PAIRS;
   hipLaunchKernelGGL(v->batched, dim3(per * members), dim3(v->TLR * v->PAIRS), 0, s, tab, per);
   return hipGetLastError();
 }
@@ -1333,15 +1371,15 @@ int rpass_variant_extra(int ny, int idx, int* W, int* PS, int* OFF, int* PK) {
   *W = v->W;
   *PS = 3;
   *OFF = 1;
-  *PK = v->PFR << 1;
+  *PK = ((v->PFR == 1) << 1) | ((v->PFR == 2) << 3);
   return 0;
 }
 
 hipError_t launch_rpass(int nx, int ny, int variant, const RPassArgs& a, hipStream_t s) {
   const RVariant* v = find_rvariant(ny, variant);
   if (!v) return hipErrorInvalidValue;
-  const int pairs = nx / 2;
-  hipLaunchKernelGGL(v->kern, dim3((pairs + v->PAIRS - 1) / v->PAIRS), dim3(v->TLR * v->PAIRS), 0, s, a);
+  const int units = v->PFR == 2 ? nx : nx / 2;   // rows or row pairs
+  hipLaunchKernelGGL(v->kern, dim3((units + v->PAIRS - 1) / v->PAIRS), dim3(v->TLR * v->PAIRS), 0, s, a);
   return hipGetLastError();
 }
 

@@ -174,8 +174,8 @@ int ns2d_variant_info(int pass, int n, int idx, int* shape4);
  * bit 0: the column pass carries the ky = 0 and ky = Ny/2 columns as one
  * packed complex line (Ny/2 column slots instead of Ny/2+1); bit 1: the
  * column pass issues its RK epilogue loads before the forward transform;
- * bit 2: the packed pair's second column stays in registers (flags are 0 for
- * the row pass). */
+ * bit 2: the packed pair's second column stays in registers; bit 3: the row
+ * pass runs one row per line group instead of a row pair (row pass only). */
 int ns2d_variant_extra(int pass, int n, int idx, int* extra4);
 /* The shape a new handle with this integrator (NS2D_INT_*) runs for a
  * transform length n unless NS2D_KPASS_VARIANT_<Nx> / NS2D_RPASS_VARIANT_<Ny>

@@ -243,6 +243,11 @@ def test_pass_variants_match_oracle(Nx, Ny, var, rvar, integ):
         s.set_state(w0)
         s.spectral_step(1e-3, 6)
         got = s.get_state()
+        mv = s.max_vort()
         K = s.nonlinear_rhs(w0)
+        wr = s.c2r(w0)
     assert rel(got, st.w_hat) < 1e-12
     assert rel(K, O.nonlinear_rhs(w0, p)) < 1e-12
+    # the row pass's vorticity transform (max|w| check, c2r)
+    assert abs(mv - O.max_vort(st.w_hat, p)) <= 1e-12 * abs(mv)
+    assert rel(wr, O.c2r(w0, Nx, Ny)) < 1e-13
