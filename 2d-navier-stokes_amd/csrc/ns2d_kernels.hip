@@ -1079,7 +1079,9 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
 // column group of the last round (-9 %), and 1024 threads with 8 rows each
 // and a 4-slot line offset beat 512 threads with 16 rows (-3 %; CN-AB4 keeps
 // the latter, kpass_default_variant).  At 1024, 512 threads with 4 rows each
-// beat 256 threads with 8 rows (-2 %, shorter per-workgroup chains).
+// beat 256 threads with 8 rows (-2 %, shorter per-workgroup chains); at 512,
+// two columns per 256 threads beat four (-12 %), at 256 eight columns per 512
+// threads beat 256 threads (-8 %).
 #define NS2D_KPASS_VARIANTS(X) \
   X(8, 16, 64, 1, 3, 1, 0, 1, 0) \
   X(8, 16, 64, 1, 3, 1, 1, 1, 0) \
@@ -1093,6 +1095,7 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(128, 16, 128, 1, 3, 1, 1, 1, 0) \
   X(128, 8, 64, 2, 3, 1, 1, 1, 0) \
   X(128, 8, 128, 2, 3, 1, 1, 1, 0) \
+  X(256, 8, 512, 2, 3, 1, 0, 1, 0) \
   X(256, 8, 256, 2, 3, 1, 0, 1, 0) \
   X(256, 8, 256, 2, 3, 1, 0, 1, 1) \
   X(256, 8, 256, 2, 3, 1, 1, 4, 0) \
@@ -1101,6 +1104,7 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(256, 8, 128, 4, 3, 1, 1, 1, 0) \
   X(256, 16, 256, 2, 3, 1, 0, 1, 0) \
   X(256, 32, 512, 2, 3, 1, 0, 1, 0) \
+  X(512, 2, 256, 2, 3, 1, 0, 1, 0) \
   X(512, 4, 256, 2, 3, 1, 0, 1, 0) \
   X(512, 4, 256, 2, 3, 1, 0, 1, 1) \
   X(512, 4, 256, 2, 3, 1, 1, 4, 0) \
@@ -1113,6 +1117,7 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(512, 8, 512, 1, 3, 1, 0, 1, 0) \
   X(512, 16, 512, 2, 3, 1, 0, 1, 0) \
   X(512, 4, 256, 2, 3, 1, 2, 1, 0) \
+  X(512, 4, 512, 2, 3, 1, 0, 1, 0) \
   X(1024, 2, 512, 2, 3, 1, 0, 1, 0) \
   X(1024, 2, 256, 2, 3, 1, 0, 1, 0) \
   X(1024, 2, 256, 2, 3, 1, 0, 1, 1) \
@@ -1271,9 +1276,10 @@ hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s) 
 // points per thread), PAIRS row pairs per workgroup, W minimum waves per SIMD,
 // PFR = 1 loads both rows' G fields up front (shapes with E = 4), PFR = 2 runs
 // one row per TLR threads instead of a pair; the first listed for each NY is
-// the default.  Single rows win at 1024 (20.5 against 21.8 us per launch: the
-// one-round grid is bound by a workgroup's chain of transforms) and lose at
-// 4096 (224 against 205 us: four rounds, where the extra transform counts).
+// the default.  Single rows win at 256 to 1024 (1024: 20.5 against 21.8 us per
+// launch, 512: 10.8 against 13.5 us; these grids run in one round and are
+// bound by a workgroup's chain of transforms), tie at 2048 and lose at 4096
+// (224 against 205 us: four rounds, where the extra transform counts).
 #define NS2D_RPASS_VARIANTS(X) \
   X(8, 1, 64, 1, 0) \
   X(16, 1, 64, 1, 0) \
@@ -1284,18 +1290,18 @@ hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s) 
   X(128, 8, 8, 1, 0) \
   X(128, 16, 4, 1, 0) \
   X(128, 32, 2, 1, 0) \
+  X(256, 64, 1, 1, 2) \
   X(256, 64, 1, 1, 0) \
   X(256, 64, 1, 1, 1) \
   X(256, 16, 4, 1, 0) \
   X(256, 32, 2, 1, 0) \
-  X(256, 64, 1, 1, 2) \
+  X(512, 64, 1, 1, 2) \
   X(512, 64, 1, 1, 0) \
   X(512, 32, 2, 1, 0) \
   X(512, 128, 1, 1, 0) \
   X(512, 128, 1, 1, 1) \
   X(512, 64, 2, 1, 0) \
   X(512, 64, 4, 1, 0) \
-  X(512, 64, 1, 1, 2) \
   X(512, 128, 1, 1, 2) \
   X(1024, 256, 1, 1, 2) \
   X(1024, 256, 1, 1, 0) \
