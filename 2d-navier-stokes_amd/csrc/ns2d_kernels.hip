@@ -723,19 +723,23 @@ __global__ __launch_bounds__(TLR * PAIRS, W) void rpass_batched(const RPassArgs*
 // rpass_adj forms the four products and their y transforms (the ky parts of
 // the multipliers and W2 applied); kpass_adj does the four x transforms, the
 // kx parts, the RK4CN adjoint recursion and r2c^T of the next stage.
+// SR = 1: TLR threads own one row (l_n of that row transformed alone, with a
+// zero imaginary part) instead of a row pair: twice the workgroups, five
+// transforms per workgroup instead of nine.
 // ---------------------------------------------------------------------------
-template <int NY, int TLR, int PAIRS>
+template <int NY, int TLR, int PAIRS, int SR>
 __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk) {
   constexpr int H = NY / 2;
   constexpr int E = NY / TLR;
   constexpr int LS = line_stride(NY);
   constexpr bool BLOCK = TLR > 64;
   static_assert(E >= 2 && (BLOCK ? PAIRS == 1 : 64 % TLR == 0), "bad rpass_adj shape");
+  static_assert(!SR || PAIRS == 1, "single-row shapes own one row per workgroup");
   __shared__ double2 lds[PAIRS * LS];
 
   const int tid = threadIdx.x;
   const int q = tid / TLR, l = tid % TLR;
-  const int x0 = 2 * (blk * PAIRS + q);
+  const int x0 = SR ? blk : 2 * (blk * PAIRS + q);
   const bool valid = x0 < a.nx;
   if (!BLOCK && !valid) return;
   const int ld = a.ld;
@@ -743,14 +747,15 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
   double2* buf = lds + q * LS;
 
   // full-length spectrum of A + iB from the half spectra of two real fields
-  auto build = [&](double2* z, const double2* fa, const double2* fb, int mula, int mulb, int lo) {
+  // (fb == nullptr only where hasb is 0: B = 0)
+  auto build = [&](double2* z, const double2* fa, const double2* fb, int mula, int mulb, int lo, int hasb = 1) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int ky = lo + TLR * e;
       const bool mirror = ky > H;
       const int kk = mirror ? NY - ky : ky;
       const double fk = (double)kk;
-      double2 A = fa[kk], B = fb[kk];
+      double2 A = fa[kk], B = hasb ? fb[kk] : make_double2(0.0, 0.0);
       if (mula) A = make_double2(-fk * A.y, fk * A.x);   // times i ky
       if (mulb) B = make_double2(-fk * B.y, fk * B.x);
       if (kk == 0 || kk == H) { A.y = 0.0; B.y = 0.0; }
@@ -784,12 +789,13 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
     xsync<BLOCK>();
   };
 
-  // l_n of rows x0 and x0+1 from the x-transformed r2c^T field L
+  // l_n of rows x0 and x0+1 (SR: of row x0) from the x-transformed r2c^T field L
   double2 zl[E];
-  build(zl, a.lin + (size_t)x0 * ld, a.lin + (size_t)(x0 + 1) * ld, 0, 0, opaque(l));
+  if constexpr (SR) build(zl, a.lin + (size_t)x0 * ld, nullptr, 0, 0, opaque(l), 0);
+  else build(zl, a.lin + (size_t)x0 * ld, a.lin + (size_t)(x0 + 1) * ld, 0, 0, opaque(l));
   fft_reg<NY, TLR, +1, BLOCK>(zl, buf, tw, opaque(l));
 #pragma unroll 1
-  for (int r = 0; r < 2; ++r) {
+  for (int r = 0; r < (SR ? 1 : 2); ++r) {
     const int x = opaque(x0 + r);
     const size_t row = (size_t)x * ld;
     double2 z1[E], z2[E];
@@ -811,9 +817,9 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
   }
 }
 
-template <int NY, int TLR, int PAIRS>
+template <int NY, int TLR, int PAIRS, int SR>
 __global__ __launch_bounds__(TLR * PAIRS) void rpass_adj_kernel(RAdjArgs a) {
-  rpass_adj_body<NY, TLR, PAIRS>(a, (int)blockIdx.x);
+  rpass_adj_body<NY, TLR, PAIRS, SR>(a, (int)blockIdx.x);
 }
 
 template <int L, int C, int T>
@@ -1425,28 +1431,32 @@ hipError_t launch_measures(int nbins, const MeasArgs& a, hipStream_t s) {
 }
 
 // adjoint pass shapes per transform length; index 0 is the default, others
-// are selectable (NS2D_KADJ_VARIANT_<N>, NS2D_RADJ_VARIANT_<N>) for tuning
+// are selectable (NS2D_KADJ_VARIANT_<N>, NS2D_RADJ_VARIANT_<N>) for tuning.
+// Row pass {NY, TLR, PAIRS, SR}: SR = 1 runs one row per TLR threads.
 #define NS2D_RADJ_VARIANTS(X) \
-  X(8, 1, 64)       \
-  X(16, 1, 64)      \
-  X(32, 4, 16)      \
-  X(64, 8, 8)       \
-  X(128, 32, 2)     \
-  X(128, 16, 4)     \
-  X(256, 64, 1)     \
-  X(256, 16, 4)     \
-  X(256, 32, 2)     \
-  X(512, 64, 1)     \
-  X(512, 128, 1)    \
-  X(512, 32, 2)     \
-  X(1024, 256, 1)   \
-  X(1024, 64, 1)    \
-  X(1024, 128, 1)   \
-  X(2048, 256, 1)   \
-  X(2048, 128, 1)   \
-  X(2048, 512, 1)   \
-  X(4096, 512, 1)   \
-  X(4096, 256, 1)  
+  X(8, 1, 64, 0)       \
+  X(16, 1, 64, 0)      \
+  X(32, 4, 16, 0)      \
+  X(64, 8, 8, 0)       \
+  X(128, 32, 2, 0)     \
+  X(128, 16, 4, 0)     \
+  X(256, 64, 1, 1)     \
+  X(256, 64, 1, 0)     \
+  X(256, 16, 4, 0)     \
+  X(256, 32, 2, 0)     \
+  X(512, 64, 1, 1)     \
+  X(512, 64, 1, 0)     \
+  X(512, 128, 1, 0)    \
+  X(512, 32, 2, 0)     \
+  X(1024, 256, 1, 1)   \
+  X(1024, 256, 1, 0)   \
+  X(1024, 64, 1, 0)    \
+  X(1024, 128, 1, 0)   \
+  X(2048, 256, 1, 0)   \
+  X(2048, 128, 1, 0)   \
+  X(2048, 512, 1, 0)   \
+  X(4096, 512, 1, 0)   \
+  X(4096, 256, 1, 0)  
 
 #define NS2D_KADJ_VARIANTS(X) \
   X(8, 16, 64)       \
@@ -1472,14 +1482,14 @@ hipError_t launch_measures(int nbins, const MeasArgs& a, hipStream_t s) {
   X(4096, 2, 512)   
 
 struct RAdjVariant {
-  int NY, TLR, PAIRS;
+  int NY, TLR, PAIRS, SR;
   void (*kern)(RAdjArgs);
 };
 struct KAdjVariant {
   int L, C, T;
   void (*kern)(KAdjArgs);
 };
-#define NS2D_RADJ_ENTRY(NY_, TLR_, PAIRS_) {NY_, TLR_, PAIRS_, rpass_adj_kernel<NY_, TLR_, PAIRS_>},
+#define NS2D_RADJ_ENTRY(NY_, TLR_, PAIRS_, SR_) {NY_, TLR_, PAIRS_, SR_, rpass_adj_kernel<NY_, TLR_, PAIRS_, SR_>},
 #define NS2D_KADJ_ENTRY(L_, C_, T_) {L_, C_, T_, kpass_adj_kernel<L_, C_, T_>},
 static const RAdjVariant radj_variants[] = {NS2D_RADJ_VARIANTS(NS2D_RADJ_ENTRY)};
 static const KAdjVariant kadj_variants[] = {NS2D_KADJ_VARIANTS(NS2D_KADJ_ENTRY)};
@@ -1510,7 +1520,8 @@ hipError_t launch_rpass_adj(int nx, int ny, const RAdjArgs& a, hipStream_t s) {
     }
   }
   if (!v) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(v->kern, dim3((nx / 2 + v->PAIRS - 1) / v->PAIRS), dim3(v->TLR * v->PAIRS), 0, s, a);
+  const int units = v->SR ? nx : nx / 2;   // rows or row pairs
+  hipLaunchKernelGGL(v->kern, dim3((units + v->PAIRS - 1) / v->PAIRS), dim3(v->TLR * v->PAIRS), 0, s, a);
   return hipGetLastError();
 }
 

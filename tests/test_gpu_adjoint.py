@@ -39,7 +39,9 @@ def enstrophy_grad(p, w):
     return g
 
 
-@pytest.mark.parametrize("Nx,Ny,n,ckpt", [(32, 32, 5, 0), (64, 32, 6, 4), (64, 64, 7, 3), (128, 128, 4, 0)])
+# Ny = 256, 512, 1024 run the single-row adjoint row pass (their default)
+@pytest.mark.parametrize("Nx,Ny,n,ckpt", [(32, 32, 5, 0), (64, 32, 6, 4), (64, 64, 7, 3), (128, 128, 4, 0),
+                                          (256, 256, 3, 2), (64, 512, 2, 0), (32, 1024, 2, 0)])
 def test_vjp_matches_torch_autograd(Nx, Ny, n, ckpt):
     print(f"vjp {Nx}x{Ny} n={n} ckpt={ckpt}", flush=True)
     p = O.Params(Nx, Ny, nu=2e-3, hyper_flag=0, hyper_pow=1.0, filter="23", quirks=0)
