@@ -80,3 +80,20 @@ def test_invalid_params_rejected_without_gpu():
     rc = lib.ns2d_create(ctypes.byref(p), ctypes.byref(h))
     assert rc == ns2d.NS2D_ERR_INVALID
     assert b"powers of two" in lib.ns2d_last_error()
+
+
+def test_default_variant_rejects_bad_arguments():
+    """ns2d_default_variant validates pass, length, integrator and its output pointer."""
+    import ctypes
+
+    import ns2d
+    lib = ns2d.load_library()
+    idx = ctypes.c_int(-7)
+    assert lib.ns2d_default_variant(ns2d.PASS_COLUMN, 100, 0, ctypes.byref(idx)) == ns2d.NS2D_ERR_INVALID
+    assert lib.ns2d_default_variant(ns2d.PASS_ROW, 8192, 0, ctypes.byref(idx)) == ns2d.NS2D_ERR_INVALID
+    assert lib.ns2d_default_variant(2, 256, 0, ctypes.byref(idx)) == ns2d.NS2D_ERR_INVALID
+    assert lib.ns2d_default_variant(ns2d.PASS_COLUMN, 256, 5, ctypes.byref(idx)) == ns2d.NS2D_ERR_INVALID
+    assert lib.ns2d_default_variant(ns2d.PASS_COLUMN, 256, 0, None) == ns2d.NS2D_ERR_INVALID
+    assert idx.value == -7
+    assert lib.ns2d_default_variant(ns2d.PASS_COLUMN, 4096, 1, ctypes.byref(idx)) == ns2d.NS2D_OK
+    assert idx.value == ns2d.default_variant(ns2d.PASS_COLUMN, 4096, "AB4") > 0
