@@ -1,10 +1,14 @@
 """Builds, in-tree: libns2d_gpu.so (HIP kernels + C ABI) for gfx950 with hipcc;
 libns2d_host.so (command line, initial conditions, HDF5 output; plain C) and
-the ns2d_solver executable (the reference-compatible driver) with gcc; the
-probes/ measurement programs with hipcc."""
+the ns2d_solver executable (the reference-compatible driver) with gcc.
+
+The kernel shapes are instantiated by ns2d_pass_tu.hip, compiled once per
+transform-length range (PASS_TUS, which must match NS2D_PASS_TUS in
+ns2d_kernels.hip); all translation units compile in parallel."""
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -12,7 +16,8 @@ LIB = os.path.join(HERE, "libns2d_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["ns2d_kernels.hip", "ns2d_capi.cpp"]
+SOURCES = ["ns2d_kernels.hip", "ns2d_adj.hip", "ns2d_capi.cpp"]
+PASS_TUS = [(8, 128), (256, 256), (512, 512), (1024, 1024), (2048, 2048), (4096, 4096)]
 DRIVER = os.path.join(HERE, "driver")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HOST_LIB = os.path.join(HERE, "libns2d_host.so")
@@ -23,16 +28,34 @@ HDF5_PREFIX = os.environ.get("NS2D_HDF5_PREFIX", "/opt/conda")
 CFLAGS = ["-O3", "-std=gnu99", "-W", "-Wall", "-fPIC"]
 
 
-def build(verbose: bool = False) -> str:
-    objs = []
+def _compile_jobs():
+    jobs = []
     for src in SOURCES:
         obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC] + FLAGS + (["-x", "hip"] if src.endswith(".cpp") else []) + \
-              ["-c", os.path.join(CSRC, src), "-o", obj]
+        jobs.append([HIPCC] + FLAGS + (["-x", "hip"] if src.endswith(".cpp") else []) +
+                    ["-c", os.path.join(CSRC, src), "-o", obj])
+    for lo, hi in PASS_TUS:
+        obj = os.path.join(CSRC, f"ns2d_pass_tu_{lo}_{hi}.o")
+        jobs.append([HIPCC] + FLAGS + [f"-DNS2D_TU_LO={lo}", f"-DNS2D_TU_HI={hi}", "-c",
+                                       os.path.join(CSRC, "ns2d_pass_tu.hip"), "-o", obj])
+    return jobs
+
+
+def build(verbose: bool = False, jobs: int = 0) -> str:
+    cmds = _compile_jobs()
+    # the largest units first, so that they do not finish last
+    cmds.sort(key=lambda c: ("ns2d_pass_tu" not in c[-1], c[-1]))
+    objs = [c[-1] for c in cmds]
+    nj = jobs or max(1, min(len(cmds), os.cpu_count() or 1, 16))
+
+    def run(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    with ThreadPoolExecutor(max_workers=nj) as ex:
+        for f in [ex.submit(run, c) for c in cmds]:
+            f.result()
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -40,17 +63,7 @@ def build(verbose: bool = False) -> str:
     for o in objs:
         os.remove(o)
     build_driver(verbose)
-    build_probes(verbose)
     return LIB
-
-
-def build_probes(verbose: bool = False) -> None:
-    """Stand-alone measurement programs under probes/ (not part of the library)."""
-    src = os.path.join(HERE, "probes", "column_bandwidth.hip")
-    cmd = [HIPCC] + FLAGS + [src, "-o", os.path.splitext(src)[0]]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
 
 
 def build_driver(verbose: bool = False) -> str:

@@ -1,0 +1,1256 @@
+// Device code of the fused HIP kernels (gfx950, FP64), shared by the
+// translation units that instantiate the pass shapes (ns2d_pass_tu.hip, built
+// once per transform-length range) and the launchers (ns2d_kernels.hip).
+#pragma once
+// Fused HIP kernels for the pseudospectral vorticity RHS and its RK4CN / AB4
+// integrators (gfx950, FP64).
+//
+// Spectral arrays live in HBM in the reference's layout [Nx][ld] of double2,
+// kx rows (0..Nx/2, -Nx/2+1..-1), ky columns 0..Ny/2 (ld >= Ny/2+1, padding
+// columns are kept at zero).  One right-hand side evaluation
+// (reference NonlinearRHSBatch, Solver/src/solver.c:884-1032) is two passes:
+//
+//   kpass  (column pass, one workgroup per group of C ky-columns)
+//          [FWD] x-forward FFT of N(x,ky) -> K = (1/(NxNy))^2 * r2c(u.grad w),
+//                (0,0)=0, dealias factor, Kolmogorov forcing  (solver.c:998-1027)
+//          [EPI] integrator stage epilogue: RK4CN stage sums and update
+//                (solver.c:690,729,768,853) or CN-AB4 update (solver.c:333),
+//                with the ForceConjugacy global write (solver.c:2089-2101)
+//          [INV] four x-inverse FFTs of the next stage input S:
+//                G1 = X^-1[S/k^2], G2 = X^-1[-i kx S/k^2], G3 = X^-1[i kx S],
+//                G4 = X^-1[S]   (solver.c:904-957 before the y transform)
+//   rpass  (row pass, one workgroup per x row)
+//          u = c2r_y(i ky G1), v = c2r_y(G2), w_x = c2r_y(G3), w_y = c2r_y(i ky G4)
+//          via two complex inverse FFTs (u+iv, w_x+iw_y); n = u w_x + v w_y
+//          (solver.c:981, note the + sign); N = r2c_y(n) via a half-length
+//          complex FFT; optionally w = c2r_y(G4) for max|w| (solver.c:2219-2291)
+//
+// The c2r_y here ignores the imaginary parts of the ky=0 and ky=Ny/2 entries,
+// which is FFTW's c2r behaviour on a non-Hermitian input column.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ns2d_fft.hpp"
+#include "ns2d_fft_reg.hpp"
+#include "ns2d_internal.h"
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+namespace ns2d {
+
+__device__ __forceinline__ int kx_of_row(int i, int nx) { return i <= nx / 2 ? i : i - nx; }
+
+// Returns v unchanged but opaque to the optimiser.  Used at the top of the
+// per-field / per-batch loops so that index and twiddle arithmetic is redone in
+// each iteration instead of being hoisted and kept live across the loop (which
+// otherwise exhausts the register file and spills to scratch).
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+
+
+// ApplyDealiasing multiplier, solver.c:1039-1099
+__device__ __forceinline__ double dealias_factor(const PhysArgs& ph, int kx, int ky) {
+  const double k2 = (double)(kx * kx + ky * ky);
+  switch (ph.filter) {
+    case FILTER_23:
+      if (k2 >= ph.kmax_sqr) return 0.0;
+      if (ph.kolm_kill && kx == 0) return 0.0;
+      return 1.0;
+    case FILTER_HOULI_REF: {
+      // integer division k/(N/2) (truncating): with kx in (-Nx/2, Nx/2] and
+      // ky in [0, Ny/2] the quotients are 1 on the Nyquist row/column and 0
+      // elsewhere, so only those differ from 1; the two possible values come
+      // precomputed from the host
+      const int q = (kx == ph.nx / 2) + (ky == ph.ny / 2);
+      return q == 0 ? 1.0 : (q == 1 ? ph.houli_edge : ph.houli_corner);
+    }
+    case FILTER_HOULI:   // exp(-36 (|k|/k_N)^36), tabulated on the host
+      return ph.filt_tab[(kx < 0 ? kx + ph.nx : kx) * ph.ld + ky];
+    default: return 1.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// kpass: column pass in k-space
+//
+// A work item is a group of C adjacent column slots.  Thread tid owns slot
+// c = tid % C and the rows x = l + TL*e (l = tid / C, TL = T/C, e < E = L/TL),
+// so each global access of a wave covers C contiguous complex values of a row,
+// and the column's x-transform runs in registers (fft_reg) with the slot's own
+// LDS exchange buffer.  KS workgroups share a column group: each redoes the
+// forward transform and takes 4/KS of the inverse fields; only split 0 writes
+// the integrator state (the new state goes to a separate buffer w_out, so no
+// workgroup reads what another writes).
+//
+// PK (packed pair): slot s >= 1 is column ky = s and slot 0 carries both the
+// ky = 0 and the ky = Ny/2 = H column, so Ny/2 slots cover the Ny/2+1 columns
+// (an even count: no work item is left holding the odd column alone).  Two
+// columns go through one complex transform in both directions:
+//   forward  the row pass leaves N(x,0) and N(x,H) real (they are the
+//            y-transforms' k = 0 and k = H terms of real rows), so
+//            Z = X[N(.,0) + i N(.,H)] gives K0 = (Z + conj Z(-kx))/2 and
+//            KH = (Z - conj Z(-kx))/(2i);
+//   inverse  the row pass uses one real number per entry of these columns:
+//            after its i ky factor it keeps the real part (FFTW's c2r
+//            projection), i.e. Re G2, Re G3, Re G4 at ky = 0 and H, Im G1 and
+//            Im G4 at H (-H Im G = Re(i H G)), nothing of G1 at ky = 0.  With
+//            Re X^-1[P] = X^-1[(P + conj P(-kx))/2], one transform of
+//            (P + conj P(-kx))/2 + i (Q + conj Q(-kx))/2 gives Re X^-1[P] and
+//            Re X^-1[Q]; the four transforms are
+//              G1 slot: P = -i psi_H, Q = -i S_H  -> Im G1(x,H), Im G4(x,H)
+//              G2 slot: P = -i kx psi_0, Q = -i kx psi_H  -> Re G2(x,0), Re G2(x,H)
+//              G3 slot: P = i kx S_0, Q = i kx S_H        -> Re G3(x,0), Re G3(x,H)
+//              G4 slot: P = S_0, Q = S_H                  -> Re G4(x,0), Re G4(x,H)
+//            (G4(x,H) is written as two 8-byte halves from two slots; the
+//            unused parts are written as zeros.)
+// With PK = 1 the columns the transforms need wait in pairbuf ([split][2][Nx],
+// per workgroup split): KH between the forward transform and its epilogue,
+// then the next stage inputs S_0 and S_H.  With PK = 2 they stay in registers
+// (E more per thread), so slot 0 costs only the extra LDS exchanges.
+// ---------------------------------------------------------------------------
+// PS: LDS padding (one slot per 2^PS elements); OFF: extra 16-B slots between
+// the slots' lines, which sets where the neighbouring slot's accesses of one
+// wave fall among the LDS banks (adjacent lanes work on adjacent slots)
+template <int L, int C, int T, int KS, int PS, int OFF, int PK, int PF>
+__device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
+  constexpr int TL = T / C;        // threads per column
+  constexpr int E = L / TL;        // rows per thread
+  constexpr int LS = padx_s<PS>(L) + OFF;
+  constexpr int FPS = 4 / KS;      // inverse fields per split
+  constexpr bool BLOCK = T > 64;
+  static_assert(T % C == 0 && L % TL == 0 && E >= 2 && 4 % KS == 0, "bad kpass shape");
+  __shared__ double2 lds[C * LS];
+
+  const int tid = threadIdx.x;
+  const int g = wi / KS, split = wi % KS;
+  const int c = tid % C, l = tid / C;
+  const int ky = g * C + c;        // the slot; with PK slot 0 is the (0, Ny/2) pair
+  const int nyf = a.nyf, ld = a.ld, nx = L;
+  const int H = nyf - 1;           // ky of the Nyquist column
+  const bool colv = ky < (PK ? H : nyf);
+  const bool pair = PK && ky == 0;
+  const bool pblk = PK && g == 0;  // uniform per workgroup
+  const bool writer = split == 0;
+  const double2* __restrict__ tw = a.tw;
+  const PhysArgs& ph = a.ph;
+  double2* buf = lds + c * LS;
+  // the pair's held columns (see above)
+  double2* __restrict__ hold = a.pairbuf + (size_t)split * 2 * L;
+
+  double2 R[E];  // forward result K, then the next stage input S
+  // PK = 2: the pair's second column (KH, then S_H) beside R; S_0 moves here
+  // when the second column's epilogue runs (see the column loop)
+  constexpr bool PKR = PK == 2;
+  double2 RH[PKR ? E : 1];
+#pragma unroll
+  for (int e = 0; e < (PKR ? E : 1); ++e) RH[e] = make_double2(0.0, 0.0);
+
+  // PF: the RK epilogue's state, accumulator and coefficient loads are issued
+  // before the forward transform, so that their latency hides behind it
+  // (unpacked shapes whose register budget leaves room for 2.5 E values)
+  static_assert(!(PF && PK), "prefetch is for unpacked shapes");
+  constexpr int EP = PF ? E : 1;
+  double2 wpre[EP], apre[EP];
+  double lpre[EP];
+  // (every epilogue that reads the state runs after a forward transform on a
+  // valid column, exactly when `pre` holds; the arrays are filled
+  // unconditionally so that they stay in registers)
+  const bool pre = PF && (a.mode & KP_FWD) && colv;
+  if constexpr (PF != 0) {
+    {
+      const bool cf = (a.mode & KP_CONJFIX) && ky == 0;
+      const bool need_acc = a.epi == EPI_RK4CN || ((a.epi == EPI_RK2 || a.epi == EPI_RK3) && writer);
+      const bool need_lam = a.epi == EPI_RK4CN;
+#pragma unroll
+      for (int it = 0; it < E; ++it) {
+        const int i = l + TL * it;
+        const int kx = kx_of_row(i, nx);
+        const size_t idx = (size_t)i * ld + ky;
+        wpre[it] = !pre ? make_double2(0.0, 0.0) : (cf && kx < 0) ? cconj(a.conj_src[-kx]) : a.w_in[idx];
+        apre[it] = (pre && need_acc) ? a.acc[idx] : make_double2(0.0, 0.0);
+        lpre[it] = (pre && need_lam) ? a.lam[idx] : 0.0;
+      }
+    }
+  }
+
+  // ---------------- FWD: x-forward FFT of the row-pass output -------------
+  if (a.mode & KP_FWD) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int x = l + TL * e;
+      if (pair) R[e] = make_double2(a.nin[(size_t)x * ld].x, a.nin[(size_t)x * ld + H].x);
+      else R[e] = colv ? a.nin[(size_t)x * ld + ky] : make_double2(0.0, 0.0);
+      if ((e & 3) == 3) asm volatile("" ::: "memory");
+    }
+    fft_reg<L, TL, -1, BLOCK, PS>(R, buf, tw, opaque(l));
+    if (pblk) {
+      if (pair) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) buf[padx_s<PS>(l + TL * e)] = R[e];
+      }
+      xsync<BLOCK>();
+      if (pair) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int x = l + TL * e;
+          const double2 z = R[e], zm = buf[padx_s<PS>((L - x) & (L - 1))];
+          R[e] = make_double2(0.5 * (z.x + zm.x), 0.5 * (z.y - zm.y));    // K0
+          const double2 kh = make_double2(0.5 * (z.y + zm.y), -0.5 * (z.x - zm.x));  // KH
+          if constexpr (PKR) RH[e] = kh;
+          else hold[L + x] = kh;
+        }
+      }
+      xsync<BLOCK>();
+    }
+  }
+
+  // ---------------- EPI: integrator stage, per column of the slot ----------
+  // The state w as the epilogue sees it, with the ForceConjugacy write of the
+  // RHS just finished: w_hat[kx<0, 0] = conj(stage_input[|kx|, 0])
+  // (solver.c:2096-2102).  Padding columns (ky >= Ny/2+1) carry zeros.
+#pragma unroll 1
+  for (int q = 0; q < (pair ? 2 : 1); ++q) {
+    // opaque: keeps the row offsets from being hoisted out of the loop and
+    // held live across it
+    const int kq = opaque(q ? H : ky);
+    const int lq = opaque(l);
+    auto index_of = [&](int it) -> size_t { return (size_t)(lq + TL * it) * ld + kq; };
+    if (PKR && q == 1) {
+      // S_0 to RH, KH (forward modes) to R
+#pragma unroll
+      for (int e = 0; e < (PKR ? E : 1); ++e) {
+        const double2 t = R[e];
+        R[e] = RH[e];
+        RH[e] = t;
+      }
+    }
+    if (a.mode & KP_FWD) {
+      if (!PKR && q == 1) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) R[e] = hold[L + lq + TL * e];
+      }
+      const double nrm = ph.norm2;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int kx = kx_of_row(lq + TL * e, nx);
+        double2 v = R[e];
+        if (a.mode & KP_RAW) {
+          // plain unnormalised forward transform (initial conditions)
+        } else if (kx == 0 && kq == 0) {
+          v = make_double2(0.0, 0.0);
+        } else {
+          v = cscale(nrm, v);                                  // solver.c:1005
+          v = cscale(dealias_factor(ph, kx, kq), v);           // solver.c:1019
+          if (ph.force_on && kx == 0 && kq == ph.force_k)      // solver.c:1022-1027
+            v.x += ph.force_val;
+        }
+        R[e] = v;
+      }
+    }
+    const bool conj_fix = (a.mode & KP_CONJFIX) && kq == 0;
+    auto state_at = [&](int it) -> double2 {
+      if constexpr (PF != 0) return wpre[it];
+      const int i = lq + TL * it;
+      const int kx = kx_of_row(i, nx);
+      if (conj_fix && kx < 0) return cconj(a.conj_src[-kx]);
+      return a.w_in[(size_t)i * ld + kq];
+    };
+    auto acc_at = [&](int it) -> double2 {
+      if constexpr (PF != 0) return apre[it];
+      return a.acc[index_of(it)];
+    };
+    auto lam_at = [&](int it) -> double {
+      if constexpr (PF != 0) return lpre[it];
+      return a.lam[index_of(it)];
+    };
+    switch (a.epi) {
+      case EPI_FROM_STATE:
+#pragma unroll
+        for (int it = 0; it < E; ++it) {
+          R[it] = colv ? a.w_in[index_of(it)] : make_double2(0.0, 0.0);
+          if ((it & 3) == 3) asm volatile("" ::: "memory");
+        }
+        break;
+      case EPI_FROM_INPUT:
+#pragma unroll
+        for (int it = 0; it < E; ++it) {
+          R[it] = colv ? a.sin[index_of(it)] : make_double2(0.0, 0.0);
+          if ((it & 3) == 3) asm volatile("" ::: "memory");
+        }
+        break;
+      case EPI_RHS:
+        if (writer && colv) {
+#pragma unroll
+          for (int it = 0; it < E; ++it) {
+            a.kout[index_of(it)] = R[it];
+            const int kx = kx_of_row(lq + TL * it, nx);
+            if (conj_fix && kx < 0) a.w_out[index_of(it)] = cconj(a.conj_src[-kx]);
+          }
+        }
+        break;
+      case EPI_RK1:
+        if (colv) {
+#pragma unroll
+          for (int it = 0; it < E; ++it) {
+            const double2 k = R[it];
+            if (writer) {
+              a.acc[index_of(it)] = cscale(RK4_B1, k);
+              if (a.hist_store) a.hist_store[index_of(it)] = k;
+            }
+            R[it] = cadd(state_at(it), cscale(a.coef, k));     // solver.c:690
+            if ((it & 3) == 3) asm volatile("" ::: "memory");
+          }
+        }
+        break;
+      case EPI_RK2:
+      case EPI_RK3:
+        if (colv) {
+          const double b = (a.epi == EPI_RK2) ? RK4_B2 : RK4_B3;
+#pragma unroll
+          for (int it = 0; it < E; ++it) {
+            const double2 k = R[it];
+            if (writer) a.acc[index_of(it)] = cadd(acc_at(it), cscale(b, k));
+            R[it] = cadd(state_at(it), cscale(a.coef, k));     // solver.c:729,768
+            if ((it & 3) == 3) asm volatile("" ::: "memory");
+          }
+        }
+        break;
+      case EPI_RK4CN:
+        if (colv) {
+#pragma unroll
+          for (int it = 0; it < E; ++it) {
+            const int kx = kx_of_row(lq + TL * it, nx);
+            double2 nw = make_double2(0.0, 0.0);               // (0,0) mode: solver.c:860
+            if (kx != 0 || kq != 0) {
+              const double2 sum = cadd(acc_at(it), cscale(RK4_B4, R[it]));
+              const double D = a.dt * lam_at(it);
+              nw = cadd(cscale((2.0 - D) / (2.0 + D), state_at(it)), cscale(2.0 * a.dt / (2.0 + D), sum));  // :853
+            }
+            if (writer) a.w_out[index_of(it)] = nw;
+            R[it] = nw;
+            if ((it & 3) == 3) asm volatile("" ::: "memory");
+          }
+        }
+        break;
+      case EPI_AB4:
+        if (colv) {
+#pragma unroll
+          for (int it = 0; it < E; ++it) {
+            const int kx = kx_of_row(lq + TL * it, nx);
+            const size_t idx = index_of(it);
+            const double2 k = R[it];
+            double2 nw = make_double2(0.0, 0.0);               // (0,0) mode: solver.c:337
+            if (kx != 0 || kq != 0) {
+              const double2 sum = cadd(cadd(cadd(cscale(AB4_1, k), cscale(AB4_2, a.h_new[idx])),
+                                            cscale(AB4_3, a.h_mid[idx])), cscale(AB4_4, a.h_old[idx]));
+              const double D = a.dt * a.lam[idx];
+              nw = cadd(cscale((2.0 - D) / (2.0 + D), state_at(it)), cscale(2.0 * a.dt / (2.0 + D), sum));  // :333
+            }
+            if (writer) {
+              a.h_store[idx] = k;   // newest history entry (solver.c:346-348)
+              a.w_out[idx] = nw;
+            }
+            R[it] = nw;
+            if ((it & 3) == 3) asm volatile("" ::: "memory");
+          }
+        }
+        break;
+      default: break;
+    }
+    if (!colv) {
+#pragma unroll
+      for (int it = 0; it < E; ++it) R[it] = make_double2(0.0, 0.0);
+    }
+    // ky=0 column of the next stage input, for the next ForceConjugacy
+    if ((a.mode & KP_INV) && writer && kq == 0 && a.conj_dst) {
+#pragma unroll
+      for (int it = 0; it < E; ++it) a.conj_dst[lq + TL * it] = R[it];
+    }
+    if (!PKR && pair && (a.mode & KP_INV)) {
+#pragma unroll
+      for (int it = 0; it < E; ++it) hold[q * L + lq + TL * it] = R[it];
+    }
+  }
+
+  // blow-up bookkeeping for the reference's per-step SystemCheck
+  // (solver.c:2301-2326): the stage-1 row pass left max|w| of the state that
+  // this step started from in stat[0]; record it and reset the accumulator.
+  if ((a.mode & KP_STAT) && wi == 0 && tid == 0) {
+    unsigned long long* st = a.stat;
+    const unsigned long long m = st[STAT_WMAX_CUR];
+    st[STAT_WMAX_LAST] = m;
+    // a NaN maximum also counts (the reference's >= test would let it pass)
+    if (!(__longlong_as_double((long long)m) < MAX_VORT_LIM) && st[STAT_FIRST_BAD] == ~0ull)
+      st[STAT_FIRST_BAD] = st[STAT_ITER];
+    st[STAT_WMAX_CUR] = 0ull;
+  }
+  if ((a.mode & KP_ITER) && wi == 0 && tid == 0) a.stat[STAT_ITER] += 1ull;
+
+  if (!(a.mode & KP_INV)) return;
+
+  // ---------------- INV: x-inverse FFTs of the derived fields ---------------
+  // f = 0: psi_hat = S/k^2, 1: -i kx psi_hat, 2: i kx S, 3: S
+  auto derive = [](int f, double2 s, int kx, int kyv) -> double2 {
+    const int k2i = kx * kx + kyv * kyv;
+    const double inv = k2i ? 1.0 / (double)k2i : 0.0;
+    if (f == 0) return cscale(inv, s);
+    if (f == 1) return make_double2(kx * inv * s.y, -(kx * inv) * s.x);
+    if (f == 2) return make_double2(-kx * s.y, kx * s.x);
+    return s;
+  };
+#pragma unroll 1
+  for (int fi = 0; fi < FPS; ++fi) {
+    const int f = split * FPS + fi;
+    const int l_o = opaque(l), ky_o = opaque(ky);
+    double2 F[E];
+    if (pblk) {
+      if (pair) {
+        // Y = P + i Q in F, V = conj P + i conj Q to the line buffer
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int x = l_o + TL * e;
+          const int kx = kx_of_row(x, nx);
+          // S_0 and S_H: after the column loop RH holds S_0 and R holds S_H (PK = 2)
+          const double2 s0 = PKR ? RH[PKR ? e : 0] : hold[x];
+          const double2 sh = PKR ? R[e] : hold[L + x];
+          double2 P, Q;
+          if (f == 0) {
+            const double2 ps = derive(0, sh, kx, H);
+            P = make_double2(ps.y, -ps.x);
+            Q = make_double2(sh.y, -sh.x);
+          } else {
+            P = derive(f, s0, kx, 0);
+            Q = derive(f, sh, kx, H);
+          }
+          F[e] = make_double2(P.x - Q.y, P.y + Q.x);
+          buf[padx_s<PS>(x)] = make_double2(P.x + Q.y, Q.x - P.y);
+        }
+      }
+      xsync<BLOCK>();
+      if (pair) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const double2 vm = buf[padx_s<PS>((L - (l_o + TL * e)) & (L - 1))];
+          F[e] = make_double2(0.5 * (F[e].x + vm.x), 0.5 * (F[e].y + vm.y));
+        }
+      }
+      xsync<BLOCK>();
+    }
+    if (!pair) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) F[e] = derive(f, R[e], kx_of_row(l_o + TL * e, nx), ky_o);
+    }
+    fft_reg<L, TL, +1, BLOCK, PS>(F, buf, tw, l_o);
+    double2* __restrict__ gout = a.g[f];
+    if (pair) {
+      double* __restrict__ g4 = reinterpret_cast<double*>(a.g[3]);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const size_t r = (size_t)(l_o + TL * e) * ld;
+        if (f == 0) {
+          gout[r] = make_double2(0.0, 0.0);
+          gout[r + H] = make_double2(0.0, F[e].x);
+          g4[2 * (r + H) + 1] = F[e].y;
+        } else if (f == 3) {
+          gout[r] = make_double2(F[e].x, 0.0);
+          g4[2 * (r + H)] = F[e].y;
+        } else {
+          gout[r] = make_double2(F[e].x, 0.0);
+          gout[r + H] = make_double2(F[e].y, 0.0);
+        }
+      }
+    } else if (colv) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) gout[(size_t)(l_o + TL * e) * ld + ky_o] = F[e];
+    }
+  }
+}
+
+// W: minimum waves per SIMD the register allocation must allow (1 = no limit)
+template <int L, int C, int T, int KS, int PS, int OFF, int PK, int W, int PF>
+__global__ __launch_bounds__(T, W) void kpass_kernel(KPassArgs a) {
+  kpass_body<L, C, T, KS, PS, OFF, PK, PF>(a, xcd_remap((int)blockIdx.x, (int)gridDim.x));
+}
+
+// All members of an ensemble size class in one launch: work items
+// [m * per_member, (m+1) * per_member) belong to member m, whose arguments
+// are tab[m] (uniform per workgroup, read with scalar loads).
+template <int L, int C, int T, int KS, int PS, int OFF, int PK, int W, int PF>
+__global__ __launch_bounds__(T, W) void kpass_batched(const KPassArgs* __restrict__ tab, int per_member) {
+  const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int m = b / per_member;
+  kpass_body<L, C, T, KS, PS, OFF, PK, PF>(tab[m], b - m * per_member);
+}
+
+// ---------------------------------------------------------------------------
+// rpass: row pass in (x, ky) space, one row pair (x0, x0+1) per TLR threads
+//
+// Thread l of a pair owns y (or ky) = l + TLR*e, e < E = NY/TLR.  For each row
+// the full-length spectrum of (A + iB) is assembled from the stored half
+// spectra of two real fields (ky >= 0 read directly, ky < 0 as conjugates;
+// ky = 0 and NY/2 keep only their real parts), transformed in registers and
+// multiplied pointwise.  The two rows' products are packed as n(x0) + i n(x0+1)
+// and transformed forward once; N(x0,k) = (W[k] + conj W[NY-k])/2 and
+// N(x0+1,k) = (W[k] - conj W[NY-k])/(2i).
+// ---------------------------------------------------------------------------
+// PFR = 1: the four G fields of both rows are loaded at the start (2 x 4 x E
+// values per thread, for shapes with few points per thread), so that the
+// loads of the second row and of each row's second field pair overlap the
+// first transforms.  PFR = 2 (single row): TLR threads own one row x0 = blk
+// instead of a pair; the product n(x0) is transformed forward on its own
+// (imaginary part zero) and N(x0,k) = W[k] with the imaginary parts of k = 0
+// and NY/2 set to zero, as a real-input transform gives them.  Twice the
+// workgroups and three transforms per workgroup instead of five, for one
+// more forward transform per row pair.
+template <int NY, int TLR, int PAIRS, int PFR>
+__device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
+  constexpr int T = TLR * PAIRS;
+  constexpr int H = NY / 2;
+  constexpr int E = NY / TLR;
+  constexpr int LS = line_stride(NY);
+  constexpr bool BLOCK = TLR > 64;
+  constexpr bool SR = PFR == 2;
+  static_assert(E >= 2 && (BLOCK ? PAIRS == 1 : 64 % TLR == 0), "bad rpass shape");
+  static_assert(!SR || PAIRS == 1, "single-row shapes own one row per workgroup");
+  __shared__ double2 lds[PAIRS * LS];
+  __shared__ double red[(T + 63) / 64];
+
+  const int tid = threadIdx.x;
+  const int q = tid / TLR, l = tid % TLR;
+  const int x0 = SR ? blk : 2 * (blk * PAIRS + q);
+  const bool valid = x0 < a.nx;   // uniform per block when BLOCK
+  const int ld = a.ld;
+  const double2* __restrict__ tw = a.tw;
+  double2* buf = lds + q * LS;
+
+  // full-length spectrum of A + iB for row x (which: 0 -> u + iv, 1 -> w_x + i w_y,
+  // 2 -> w(x) + i w(x+1), 3 -> w(x))
+  auto build = [&](double2* z, int which, int x, int lo) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int ky = lo + TLR * e;
+      const bool mirror = ky > H;
+      const int kk = mirror ? NY - ky : ky;
+      double2 A, B;
+      const double fk = (double)kk;
+      if (which == 0) {         // u = i ky G1, v = G2
+        const double2 g1 = a.g[0][(size_t)x * ld + kk];
+        A = make_double2(-fk * g1.y, fk * g1.x);
+        B = a.g[1][(size_t)x * ld + kk];
+      } else if (which == 1) {  // w_x = G3, w_y = i ky G4
+        A = a.g[2][(size_t)x * ld + kk];
+        const double2 g4 = a.g[3][(size_t)x * ld + kk];
+        B = make_double2(-fk * g4.y, fk * g4.x);
+      } else if (which == 2) {  // w of rows x and x+1 = G4
+        A = a.g[3][(size_t)x * ld + kk];
+        B = a.g[3][(size_t)(x + 1) * ld + kk];
+      } else {                  // w of row x alone
+        A = a.g[3][(size_t)x * ld + kk];
+        B = make_double2(0.0, 0.0);
+      }
+      if (kk == 0 || kk == H) { A.y = 0.0; B.y = 0.0; }
+      z[e] = mirror ? make_double2(A.x + B.y, B.x - A.y) : make_double2(A.x - B.y, A.y + B.x);
+      if ((e & 3) == 3) asm volatile("" ::: "memory");
+    }
+  };
+
+  double n0[E];
+  double2 w[E];
+  if constexpr (PFR == 1) {
+    if ((a.mode & RP_NL) && (BLOCK || valid)) {
+      double2 pre[2][4][E];
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            const int ky = l + TLR * e;
+            const int kk = ky > H ? NY - ky : ky;
+            pre[r][f][e] = a.g[f][(size_t)(x0 + r) * ld + kk];
+          }
+      // the same assembly as build() from the loaded values
+      auto build_pre = [&](double2* z, int which, const double2 (&g)[4][E]) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int ky = l + TLR * e;
+          const bool mirror = ky > H;
+          const int kk = mirror ? NY - ky : ky;
+          const double fk = (double)kk;
+          double2 A, B;
+          if (which == 0) {
+            A = make_double2(-fk * g[0][e].y, fk * g[0][e].x);
+            B = g[1][e];
+          } else {
+            A = g[2][e];
+            B = make_double2(-fk * g[3][e].y, fk * g[3][e].x);
+          }
+          if (kk == 0 || kk == H) { A.y = 0.0; B.y = 0.0; }
+          z[e] = mirror ? make_double2(A.x + B.y, B.x - A.y) : make_double2(A.x - B.y, A.y + B.x);
+        }
+      };
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        double2 z1[E], z2[E];
+        build_pre(z1, 0, pre[r]);
+        fft_reg<NY, TLR, +1, BLOCK>(z1, buf, tw, opaque(l));
+        build_pre(z2, 1, pre[r]);
+        fft_reg<NY, TLR, +1, BLOCK>(z2, buf, tw, opaque(l));
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const double nl = z1[e].x * z2[e].x + z1[e].y * z2[e].y;   // u*w_x + v*w_y
+          if (r == 0) n0[e] = nl;
+          else w[e] = make_double2(n0[e], nl);
+        }
+      }
+    }
+  } else if (SR && (a.mode & RP_NL) && (BLOCK || valid)) {
+    double2 z1[E], z2[E];
+    build(z1, 0, x0, opaque(l));
+    fft_reg<NY, TLR, +1, BLOCK>(z1, buf, tw, opaque(l));
+    build(z2, 1, x0, opaque(l));
+    fft_reg<NY, TLR, +1, BLOCK>(z2, buf, tw, opaque(l));
+#pragma unroll
+    for (int e = 0; e < E; ++e) w[e] = make_double2(z1[e].x * z2[e].x + z1[e].y * z2[e].y, 0.0);
+  } else if ((a.mode & RP_NL) && (BLOCK || valid)) {
+#pragma unroll 1
+    for (int r = 0; r < 2; ++r) {
+      const int x = opaque(x0 + r);
+      double2 z1[E], z2[E];
+      build(z1, 0, x, opaque(l));
+      fft_reg<NY, TLR, +1, BLOCK>(z1, buf, tw, opaque(l));
+      build(z2, 1, x, opaque(l));
+      fft_reg<NY, TLR, +1, BLOCK>(z2, buf, tw, opaque(l));
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const double nl = z1[e].x * z2[e].x + z1[e].y * z2[e].y;   // u*w_x + v*w_y
+        if (r == 0) n0[e] = nl;
+        else w[e] = make_double2(n0[e], nl);
+      }
+    }
+  }
+
+  if (a.mode & RP_W) {
+    double m = 0.0;
+    if (BLOCK || valid) {
+      double2 z[E];
+      build(z, SR ? 3 : 2, x0, opaque(l));
+      fft_reg<NY, TLR, +1, BLOCK>(z, buf, tw, opaque(l));
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int y = l + TLR * e;
+        const double w0 = z[e].x * a.wscale, w1 = SR ? 0.0 : z[e].y * a.wscale;
+        if (a.wreal) {
+          a.wreal[(size_t)x0 * NY + y] = w0;
+          if (!SR) a.wreal[(size_t)(x0 + 1) * NY + y] = w1;
+        }
+        const double aw = fmax(fabs(w0), fabs(w1));
+        m = (aw > m || aw != aw) ? aw : m;
+        if (w0 != w0 || w1 != w1) m = __longlong_as_double(0x7ff8000000000000ll);
+      }
+    }
+    // wave then block max; a NaN propagates (its bit pattern compares largest)
+    unsigned long long mb = (unsigned long long)__double_as_longlong(m);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(mb, off, 64);
+      mb = o > mb ? o : mb;
+    }
+    if ((tid & 63) == 0) red[tid >> 6] = __longlong_as_double((long long)mb);
+    __syncthreads();
+    if (tid == 0) {
+      unsigned long long bm = 0;
+      for (int i = 0; i < (T + 63) / 64; ++i) {
+        const unsigned long long v = (unsigned long long)__double_as_longlong(red[i]);
+        bm = v > bm ? v : bm;
+      }
+      atomicMax(a.wmax, bm);
+    }
+  }
+
+  if (SR && (a.mode & RP_NL) && (BLOCK || valid)) {
+    fft_reg<NY, TLR, -1, BLOCK>(w, buf, tw, opaque(l));
+    double2* __restrict__ out0 = a.nout + (size_t)x0 * ld;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = l + TLR * e;
+      if (k <= H) out0[k] = (k == 0 || k == H) ? make_double2(w[e].x, 0.0) : w[e];
+    }
+  } else if ((a.mode & RP_NL) && (BLOCK || valid)) {
+    fft_reg<NY, TLR, -1, BLOCK>(w, buf, tw, opaque(l));
+    // W[NY-k] for the thread's own k: exchange through the line buffer
+#pragma unroll
+    for (int e = 0; e < E; ++e) buf[padx(l + TLR * e)] = w[e];
+    xsync<BLOCK>();
+    double2* __restrict__ out0 = a.nout + (size_t)x0 * ld;
+    double2* __restrict__ out1 = out0 + ld;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = l + TLR * e;
+      if (k <= H) {
+        const double2 p = w[e];
+        const double2 qm = cconj(buf[padx((NY - k) & (NY - 1))]);
+        out0[k] = make_double2(0.5 * (p.x + qm.x), 0.5 * (p.y + qm.y));
+        out1[k] = make_double2(0.5 * (p.y - qm.y), -0.5 * (p.x - qm.x));
+      }
+    }
+  }
+}
+
+// W: minimum waves per SIMD the register allocation must allow (1 = no limit)
+template <int NY, int TLR, int PAIRS, int W, int PFR>
+__global__ __launch_bounds__(TLR * PAIRS, W) void rpass_kernel(RPassArgs a) {
+  rpass_body<NY, TLR, PAIRS, PFR>(a, (int)blockIdx.x);
+}
+
+template <int NY, int TLR, int PAIRS, int W, int PFR>
+__global__ __launch_bounds__(TLR * PAIRS, W) void rpass_batched(const RPassArgs* __restrict__ tab, int per_member) {
+  const int m = (int)blockIdx.x / per_member;
+  rpass_body<NY, TLR, PAIRS, PFR>(tab[m], (int)blockIdx.x - m * per_member);
+}
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint (vector-Jacobian product) of the RK4CN step.
+//
+// With K = N(S) = M .* r2c(u w_x + v w_y), u = c2r(U S), v = c2r(V S),
+// w_x = c2r(X S), w_y = c2r(Y S) (U = i ky/|k|^2, V = -i kx/|k|^2, X = i kx,
+// Y = i ky, M = dealias * (1/(NxNy))^2, zero at k = 0) and gradients packed as
+// g = dJ/dRe + i dJ/dIm, the transposes are
+//   r2c^T(l) = c2r(Wh .* l),   Wh = 1/2 on interior ky columns, 1 on ky = 0, Ny/2
+//   c2r^T(f) = W2 .* r2c(f),   W2 = 2 on interior ky columns, 1 on ky = 0, Ny/2
+// so, with l_n = r2c^T(M .* l_K):
+//   N'(S)^T l_K = conj(U) W2 r2c(l_n w_x) + conj(X) W2 r2c(l_n u)
+//               + conj(V) W2 r2c(l_n w_y) + conj(Y) W2 r2c(l_n v).
+// rpass_adj forms the four products and their y transforms (the ky parts of
+// the multipliers and W2 applied); kpass_adj does the four x transforms, the
+// kx parts, the RK4CN adjoint recursion and r2c^T of the next stage.
+// SR = 1: TLR threads own one row (l_n of that row transformed alone, with a
+// zero imaginary part) instead of a row pair: twice the workgroups, five
+// transforms per workgroup instead of nine.
+// ---------------------------------------------------------------------------
+template <int NY, int TLR, int PAIRS, int SR>
+__device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk) {
+  constexpr int H = NY / 2;
+  constexpr int E = NY / TLR;
+  constexpr int LS = line_stride(NY);
+  constexpr bool BLOCK = TLR > 64;
+  static_assert(E >= 2 && (BLOCK ? PAIRS == 1 : 64 % TLR == 0), "bad rpass_adj shape");
+  static_assert(!SR || PAIRS == 1, "single-row shapes own one row per workgroup");
+  __shared__ double2 lds[PAIRS * LS];
+
+  const int tid = threadIdx.x;
+  const int q = tid / TLR, l = tid % TLR;
+  const int x0 = SR ? blk : 2 * (blk * PAIRS + q);
+  const bool valid = x0 < a.nx;
+  if (!BLOCK && !valid) return;
+  const int ld = a.ld;
+  const double2* __restrict__ tw = a.tw;
+  double2* buf = lds + q * LS;
+
+  // full-length spectrum of A + iB from the half spectra of two real fields
+  // (fb == nullptr only where hasb is 0: B = 0)
+  auto build = [&](double2* z, const double2* fa, const double2* fb, int mula, int mulb, int lo, int hasb = 1) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int ky = lo + TLR * e;
+      const bool mirror = ky > H;
+      const int kk = mirror ? NY - ky : ky;
+      const double fk = (double)kk;
+      double2 A = fa[kk], B = hasb ? fb[kk] : make_double2(0.0, 0.0);
+      if (mula) A = make_double2(-fk * A.y, fk * A.x);   // times i ky
+      if (mulb) B = make_double2(-fk * B.y, fk * B.x);
+      if (kk == 0 || kk == H) { A.y = 0.0; B.y = 0.0; }
+      z[e] = mirror ? make_double2(A.x + B.y, B.x - A.y) : make_double2(A.x - B.y, A.y + B.x);
+      if ((e & 3) == 3) asm volatile("" ::: "memory");
+    }
+  };
+  // r2c of the real and imaginary parts of a transformed packed row:
+  // P[k] = (Z[k] + conj Z[-k])/2, Q[k] = (Z[k] - conj Z[-k])/(2i); writes
+  // W2 (mul ? -i k : 1) P to pout and W2 (mulq ? -i k : 1) Q to qout
+  auto split_store = [&](double2* z, double2* pout, double2* qout, int mulp, int mulq, int lo) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) buf[padx(lo + TLR * e)] = z[e];
+    xsync<BLOCK>();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = lo + TLR * e;
+      if (k <= H) {
+        const double2 zp = z[e];
+        const double2 zm = cconj(buf[padx((NY - k) & (NY - 1))]);
+        double2 P = make_double2(0.5 * (zp.x + zm.x), 0.5 * (zp.y + zm.y));
+        double2 Q = make_double2(0.5 * (zp.y - zm.y), -0.5 * (zp.x - zm.x));
+        const double w2 = (k == 0 || k == H) ? 1.0 : 2.0;
+        const double fk = (double)k;
+        if (mulp) P = make_double2(fk * P.y, -fk * P.x);   // times -i k
+        if (mulq) Q = make_double2(fk * Q.y, -fk * Q.x);
+        pout[k] = cscale(w2, P);
+        qout[k] = cscale(w2, Q);
+      }
+    }
+    xsync<BLOCK>();
+  };
+
+  // l_n of rows x0 and x0+1 (SR: of row x0) from the x-transformed r2c^T field L
+  double2 zl[E];
+  if constexpr (SR) build(zl, a.lin + (size_t)x0 * ld, nullptr, 0, 0, opaque(l), 0);
+  else build(zl, a.lin + (size_t)x0 * ld, a.lin + (size_t)(x0 + 1) * ld, 0, 0, opaque(l));
+  fft_reg<NY, TLR, +1, BLOCK>(zl, buf, tw, opaque(l));
+#pragma unroll 1
+  for (int r = 0; r < (SR ? 1 : 2); ++r) {
+    const int x = opaque(x0 + r);
+    const size_t row = (size_t)x * ld;
+    double2 z1[E], z2[E];
+    build(z1, a.g[0] + row, a.g[1] + row, 1, 0, opaque(l));   // u + i v
+    fft_reg<NY, TLR, +1, BLOCK>(z1, buf, tw, opaque(l));
+    build(z2, a.g[2] + row, a.g[3] + row, 0, 1, opaque(l));   // w_x + i w_y
+    fft_reg<NY, TLR, +1, BLOCK>(z2, buf, tw, opaque(l));
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const double ln = (r == 0) ? zl[e].x : zl[e].y;
+      const double2 u = z1[e], wv = z2[e];
+      z1[e] = make_double2(ln * wv.x, ln * u.x);   // l_n w_x + i l_n u
+      z2[e] = make_double2(ln * wv.y, ln * u.y);   // l_n w_y + i l_n v
+    }
+    fft_reg<NY, TLR, -1, BLOCK>(z1, buf, tw, opaque(l));
+    split_store(z1, a.bout[0] + row, a.bout[1] + row, 1, 0, opaque(l));
+    fft_reg<NY, TLR, -1, BLOCK>(z2, buf, tw, opaque(l));
+    split_store(z2, a.bout[2] + row, a.bout[3] + row, 0, 1, opaque(l));
+  }
+}
+
+template <int NY, int TLR, int PAIRS, int SR>
+__global__ __launch_bounds__(TLR * PAIRS) void rpass_adj_kernel(RAdjArgs a) {
+  rpass_adj_body<NY, TLR, PAIRS, SR>(a, (int)blockIdx.x);
+}
+
+template <int L, int C, int T>
+__global__ __launch_bounds__(T) void kpass_adj_kernel(KAdjArgs a) {
+  constexpr int TL = T / C;
+  constexpr int E = L / TL;
+  constexpr int LS = line_stride(L);
+  constexpr bool BLOCK = T > 64;
+  static_assert(T % C == 0 && L % TL == 0 && E >= 2, "bad kpass_adj shape");
+  __shared__ double2 lds[C * LS];
+
+  const int tid = threadIdx.x;
+  const int g = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int ky0 = g * C;
+  const int c = tid % C, l = tid / C;
+  const int ky = ky0 + c;
+  const int nyf = a.nyf, ld = a.ld, nx = L;
+  const bool colv = ky < nyf;
+  const double2* __restrict__ tw = a.tw;
+  const PhysArgs& ph = a.ph;
+  double2* buf = lds + c * LS;
+  const double h = a.dt;
+
+  double2 LSacc[E];   // N'(S)^T l_K of the stage just finished
+#pragma unroll
+  for (int e = 0; e < E; ++e) LSacc[e] = make_double2(0.0, 0.0);
+  if (a.mode & KA_FWD) {
+#pragma unroll 1
+    for (int f = 0; f < 4; ++f) {
+      const int l_o = opaque(l), ky_o = opaque(ky);
+      double2 F[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        F[e] = colv ? a.bin[f][(size_t)(l_o + TL * e) * ld + ky_o] : make_double2(0.0, 0.0);
+        if ((e & 3) == 3) asm volatile("" ::: "memory");
+      }
+      fft_reg<L, TL, -1, BLOCK>(F, buf, tw, l_o);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int kx = kx_of_row(l_o + TL * e, nx);
+        const int k2i = kx * kx + ky_o * ky_o;
+        const double inv = k2i ? 1.0 / (double)k2i : 0.0;
+        const double2 v = F[e];
+        double2 t;
+        if (f == 0) t = cscale(inv, v);                                   // conj(U): 1/|k|^2 (-i ky in B1)
+        else if (f == 1) t = make_double2(kx * v.y, -kx * v.x);          // conj(X) = -i kx
+        else if (f == 2) t = make_double2(-(kx * inv) * v.y, (kx * inv) * v.x);  // conj(V) = i kx/|k|^2
+        else t = v;                                                       // conj(Y): -i ky in B4
+        LSacc[e] = cadd(LSacc[e], t);
+      }
+    }
+  }
+
+  // RK4CN adjoint recursion (backwards through solver.c:690-853)
+  double2 LK[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    LK[e] = make_double2(0.0, 0.0);
+    if (!colv) continue;
+    const int kx = kx_of_row(l + TL * e, nx);
+    const size_t idx = (size_t)(l + TL * e) * ld + ky;
+    const bool zero = (kx == 0 && ky == 0);
+    const double D = h * a.lam[idx];
+    const double A = (2.0 - D) / (2.0 + D), B = 2.0 * h / (2.0 + D);
+    double2 lp = zero ? make_double2(0.0, 0.0) : a.lamp_in[idx];   // dJ/dw_{n+1}
+    switch (a.epi) {
+      case ADJ_START:
+        a.lacc[idx] = cscale(A, lp);
+        LK[e] = cscale(B / 6.0, lp);
+        break;
+      case ADJ_S4:
+        a.lacc[idx] = cadd(a.lacc[idx], LSacc[e]);
+        LK[e] = cadd(cscale(B / 3.0, lp), cscale(h, LSacc[e]));
+        break;
+      case ADJ_S3:
+        a.lacc[idx] = cadd(a.lacc[idx], LSacc[e]);
+        LK[e] = cadd(cscale(B / 3.0, lp), cscale(0.5 * h, LSacc[e]));
+        break;
+      case ADJ_S2:
+        a.lacc[idx] = cadd(a.lacc[idx], LSacc[e]);
+        LK[e] = cadd(cscale(B / 6.0, lp), cscale(0.5 * h, LSacc[e]));
+        break;
+      case ADJ_S1: {
+        // dJ/dw_n; with KA_INV it starts the adjoint of the previous step
+        const double2 lw = cadd(a.lacc[idx], LSacc[e]);
+        a.lamp_out[idx] = lw;
+        if (a.mode & KA_INV) {
+          const double Dp = h * a.lam_prev[idx];
+          const double Ap = (2.0 - Dp) / (2.0 + Dp), Bp = 2.0 * h / (2.0 + Dp);
+          const double2 lq = zero ? make_double2(0.0, 0.0) : lw;
+          a.lacc[idx] = cscale(Ap, lq);
+          LK[e] = cscale(Bp / 6.0, lq);
+        }
+        break;
+      }
+      default: break;
+    }
+    if ((e & 3) == 3) asm volatile("" ::: "memory");
+  }
+  if (!(a.mode & KA_INV)) return;
+
+  // l_n transform of the next stage: x-inverse of Wh .* M .* l_K
+  {
+    const int l_o = opaque(l), ky_o = opaque(ky);
+    const double wh = (ky_o == 0 || ky_o == ph.ny / 2) ? 1.0 : 0.5;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int kx = kx_of_row(l_o + TL * e, nx);
+      double m = 0.0;
+      if (colv && !(kx == 0 && ky_o == 0)) m = wh * ph.norm2 * dealias_factor(ph, kx, ky_o);
+      LK[e] = cscale(m, LK[e]);
+    }
+    fft_reg<L, TL, +1, BLOCK>(LK, buf, tw, l_o);
+    if (colv) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) a.lout[(size_t)(l_o + TL * e) * ld + ky_o] = LK[e];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// forward transform of a real field (ICs): r2c along y per row
+// ---------------------------------------------------------------------------
+template <int NY, int T>
+__global__ __launch_bounds__(T) void r2c_rows_kernel(const double* __restrict__ in, double2* __restrict__ out,
+                                                       const double2* __restrict__ tw, int ld) {
+  constexpr int H = NY / 2;
+  constexpr int ITY = (NY + T - 1) / T;
+  constexpr int ITK = (H + 1 + T - 1) / T;
+  __shared__ double2 lds[line_stride(H)];
+  const int tid = threadIdx.x, x = blockIdx.x;
+  double* ldsr = reinterpret_cast<double*>(lds);
+#pragma unroll
+  for (int it = 0; it < ITY; ++it) {
+    const int y = tid + it * T;
+    if (y < NY) ldsr[2 * padx(y >> 1) + (y & 1)] = in[(size_t)x * NY + y];
+  }
+  __syncthreads();
+  fft_lds<H, 1, line_stride(H), T, -1>(lds, tw, tid);
+  constexpr int TWS = NS2D_TW_LEN / NY;
+#pragma unroll
+  for (int it = 0; it < ITK; ++it) {
+    const int k = tid + it * T;
+    if (k <= H) {
+      double2 X;
+      if (k == 0 || k == H) {
+        const double2 c0 = lds[0];
+        X = make_double2(k == 0 ? c0.x + c0.y : c0.x - c0.y, 0.0);
+      } else {
+        const double2 p = lds[padx(k)];
+        const double2 q = cconj(lds[padx(H - k)]);
+        const double2 E = make_double2(0.5 * (p.x + q.x), 0.5 * (p.y + q.y));
+        const double2 O = make_double2(0.5 * (p.y - q.y), -0.5 * (p.x - q.x));
+        X = cadd(E, cmul(tw[k * TWS], O));
+      }
+      out[(size_t)x * ld + k] = X;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// measures: one workgroup per spectrum bin (annulus round(|k|) = b)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ long isqrt_floor(long v) {
+  if (v < 0) return -1;
+  long r = (long)sqrt((double)v);
+  while (r * r > v) --r;
+  while ((r + 1) * (r + 1) <= v) ++r;
+  return r;
+}
+
+template <int T>
+__global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
+  __shared__ double red[NQ][T / 64];
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int nx = a.nx, ny = a.ny, H = ny / 2;
+  double acc[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+  if (b > 0) {
+    // b^2 - b < k^2 <= b^2 + b  <=>  round(|k|) == b
+    const long lo = (long)b * b - b, hi = (long)b * b + b;
+    const int kxmax = b + 1;
+    for (int kxo = tid; kxo <= 2 * kxmax; kxo += T) {
+      const int kx = kxo - kxmax;
+      if (kx < -nx / 2 + 1 || kx > nx / 2) continue;
+      const long kx2 = (long)kx * kx;
+      long ylo = (lo - kx2 < 0) ? 0 : isqrt_floor(lo - kx2) + 1;
+      long yhi = isqrt_floor(hi - kx2);
+      if (yhi > H) yhi = H;
+      const int i = kx >= 0 ? kx : kx + nx;
+      for (long ky = ylo; ky <= yhi; ++ky) {
+        const size_t idx = (size_t)i * a.ld + ky;
+        const double2 w = a.w_hat[idx];
+        const double k2 = (double)(kx2 + ky * ky);
+        const double c = (ky == 0 || ky == H) ? 1.0 : 2.0;
+        const double a2 = w.x * w.x + w.y * w.y;
+        const double pre = a.lam[idx];
+        acc[Q_ENRG] += c * a2 / k2;
+        acc[Q_ENST] += c * a2;
+        acc[Q_PALIN] += c * k2 * a2;
+        // |u|^2+|v|^2 with u = i ky w/k^2, v = -i kx w/k^2
+        const double uv2 = (double)(ky * ky) * a2 / (k2 * k2) + (double)kx2 * a2 / (k2 * k2);
+        acc[Q_ENRG_DISS] += c * pre * uv2;
+        acc[Q_ENST_DISS] += c * ((a.quirk_enst_diss && c == 2.0) ? 1.0 : pre) * a2;
+        {
+          // |i (kx u + ky v)|^2, sys_msr.c:186-193 (zero up to rounding)
+          const double2 u = make_double2(-(double)ky * w.y / k2, (double)ky * w.x / k2);
+          const double2 v = make_double2((double)kx * w.y / k2, -(double)kx * w.x / k2);
+          const double dx = (double)kx * u.x + (double)ky * v.x, dy = (double)kx * u.y + (double)ky * v.y;
+          acc[Q_DIV] += c * (dx * dx + dy * dy);
+        }
+        if (a.rhs) {
+          double2 r = a.rhs[idx];
+          // the flux terms ignore the forcing (sys_msr.c:128-132)
+          if (a.ph.force_on && kx == 0 && ky == a.ph.force_k) r.x -= a.ph.force_val;
+          const double deriv = 2.0 * (w.x * r.x + w.y * r.y);
+          const double diss = pre * a2;
+          acc[Q_DERIV] += c * deriv;
+          acc[Q_DISS] += c * diss;
+          acc[Q_DERIV_K2] += c * deriv / k2;
+          acc[Q_DISS_K2] += c * diss / k2;
+          if (k2 < a.sbst_hi && k2 >= a.sbst_lo) {
+            acc[Q_SB_DERIV] += c * deriv;
+            acc[Q_SB_DISS] += c * diss;
+            acc[Q_SB_DERIV_K2] += c * deriv / k2;
+            acc[Q_SB_DISS_K2] += c * diss / k2;
+          }
+        }
+      }
+    }
+  }
+  // deterministic block reduction: wave shuffles, then a fixed-order sum
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double v = acc[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((tid & 63) == 0) red[q][tid >> 6] = v;
+  }
+  __syncthreads();
+  if (tid < NQ) {
+    double s = 0.0;
+    for (int w = 0; w < T / 64; ++w) s += red[tid][w];
+    a.out[(size_t)b * NQ + tid] = s;
+  }
+}
+
+
+// kpass variants {L, C, T, KS, PS, OFF, PK, W, PF}: L-point columns, C column
+// slots per workgroup, T threads, KS workgroups per column group, LDS padding
+// shift PS and line offset OFF, PK = 1 packs the ky = 0 and ky = Ny/2 columns
+// into slot 0 (PK = 2: with the pair's second column in registers), W minimum
+// waves per SIMD, PF = 1 issues the RK epilogue's loads before the forward
+// transform; the first listed for each L is the default.  Rows per thread
+// E = L*C/T.  Measured on square grids: up to L = 2048 the unpacked shape is
+// faster (the packed pair's extra exchanges and held columns make slot 0 the
+// last workgroup to finish, and at 2048 its register budget costs
+// occupancy); at 4096, one workgroup per CU, the packed pair removes the odd
+// column group of the last round (-9 %), and 1024 threads with 8 rows each
+// and a 4-slot line offset beat 512 threads with 16 rows (-3 %; CN-AB4 keeps
+// the latter, kpass_default_variant).  At 1024, 512 threads with 4 rows each
+// beat 256 threads with 8 rows (-2 %, shorter per-workgroup chains); at 512,
+// two columns per 256 threads beat four (-12 %), at 256 eight columns per 512
+// threads beat 256 threads (-8 %).
+#define NS2D_KPASS_VARIANTS(X) \
+  X(8, 16, 64, 1, 3, 1, 0, 1, 0) \
+  X(8, 16, 64, 1, 3, 1, 1, 1, 0) \
+  X(16, 16, 64, 1, 3, 1, 0, 1, 0) \
+  X(16, 16, 64, 1, 3, 1, 1, 1, 0) \
+  X(32, 16, 64, 1, 3, 1, 0, 1, 0) \
+  X(32, 16, 64, 1, 3, 1, 1, 1, 0) \
+  X(64, 16, 64, 1, 3, 1, 0, 1, 0) \
+  X(64, 16, 64, 1, 3, 1, 1, 1, 0) \
+  X(128, 16, 128, 1, 3, 1, 0, 1, 0) \
+  X(128, 16, 128, 1, 3, 1, 1, 1, 0) \
+  X(128, 8, 64, 2, 3, 1, 1, 1, 0) \
+  X(128, 8, 128, 2, 3, 1, 1, 1, 0) \
+  X(256, 8, 512, 2, 3, 1, 0, 1, 0) \
+  X(256, 8, 256, 2, 3, 1, 0, 1, 0) \
+  X(256, 8, 256, 2, 3, 1, 0, 1, 1) \
+  X(256, 8, 256, 2, 3, 1, 1, 4, 0) \
+  X(256, 16, 256, 1, 3, 1, 1, 1, 0) \
+  X(256, 8, 128, 2, 3, 1, 1, 1, 0) \
+  X(256, 8, 128, 4, 3, 1, 1, 1, 0) \
+  X(256, 16, 256, 2, 3, 1, 0, 1, 0) \
+  X(256, 32, 512, 2, 3, 1, 0, 1, 0) \
+  X(512, 2, 256, 2, 3, 1, 0, 1, 0) \
+  X(512, 4, 256, 2, 3, 1, 0, 1, 0) \
+  X(512, 4, 256, 2, 3, 1, 0, 1, 1) \
+  X(512, 4, 256, 2, 3, 1, 1, 4, 0) \
+  X(512, 8, 256, 2, 3, 1, 1, 1, 0) \
+  X(512, 8, 256, 1, 3, 1, 1, 1, 0) \
+  X(512, 4, 128, 4, 3, 1, 1, 1, 0) \
+  X(512, 8, 256, 4, 3, 1, 1, 1, 0) \
+  X(512, 8, 512, 2, 3, 1, 1, 1, 0) \
+  X(512, 8, 512, 2, 3, 1, 0, 1, 0) \
+  X(512, 8, 512, 1, 3, 1, 0, 1, 0) \
+  X(512, 16, 512, 2, 3, 1, 0, 1, 0) \
+  X(512, 4, 256, 2, 3, 1, 2, 1, 0) \
+  X(512, 4, 512, 2, 3, 1, 0, 1, 0) \
+  X(1024, 2, 512, 2, 3, 1, 0, 1, 0) \
+  X(1024, 2, 256, 2, 3, 1, 0, 1, 0) \
+  X(1024, 2, 256, 2, 3, 1, 0, 1, 1) \
+  X(1024, 2, 256, 2, 3, 1, 1, 1, 0) \
+  X(1024, 2, 256, 2, 3, 1, 1, 4, 0) \
+  X(1024, 2, 256, 2, 3, 4, 1, 1, 0) \
+  X(1024, 2, 256, 2, 3, 8, 1, 1, 0) \
+  X(1024, 4, 256, 2, 3, 1, 1, 1, 0) \
+  X(1024, 4, 256, 1, 3, 1, 1, 1, 0) \
+  X(1024, 4, 256, 4, 3, 1, 1, 1, 0) \
+  X(1024, 2, 128, 2, 3, 1, 1, 1, 0) \
+  X(1024, 2, 128, 4, 3, 1, 1, 1, 0) \
+  X(1024, 1, 64, 4, 3, 1, 1, 1, 0) \
+  X(1024, 4, 512, 2, 3, 1, 1, 1, 0) \
+  X(1024, 4, 512, 1, 3, 1, 1, 1, 0) \
+  X(1024, 8, 512, 4, 4, 4, 1, 1, 0) \
+  X(1024, 8, 1024, 4, 3, 4, 1, 1, 0) \
+  X(1024, 8, 1024, 2, 3, 4, 1, 1, 0) \
+  X(1024, 4, 256, 1, 3, 1, 0, 1, 0) \
+  X(1024, 8, 512, 2, 3, 1, 0, 1, 0) \
+  X(1024, 8, 512, 1, 3, 1, 0, 1, 0) \
+  X(1024, 8, 1024, 1, 3, 1, 0, 1, 0) \
+  X(1024, 4, 512, 1, 3, 1, 0, 1, 0) \
+  X(1024, 4, 512, 2, 3, 1, 0, 1, 0) \
+  X(1024, 2, 256, 2, 3, 1, 2, 1, 0) \
+  X(1024, 2, 256, 2, 3, 1, 2, 3, 0) \
+  X(1024, 2, 256, 4, 3, 1, 0, 1, 0) \
+  X(1024, 2, 512, 4, 3, 1, 0, 1, 0) \
+  X(2048, 2, 512, 1, 3, 1, 0, 1, 0) \
+  X(2048, 2, 512, 1, 3, 1, 0, 1, 1) \
+  X(2048, 2, 512, 1, 3, 1, 1, 4, 0) \
+  X(2048, 2, 256, 2, 3, 1, 1, 1, 0) \
+  X(2048, 2, 256, 1, 3, 1, 1, 1, 0) \
+  X(2048, 4, 512, 1, 3, 1, 1, 1, 0) \
+  X(2048, 1, 128, 4, 3, 1, 1, 1, 0) \
+  X(2048, 4, 1024, 1, 3, 1, 1, 1, 0) \
+  X(2048, 2, 512, 1, 3, 1, 2, 1, 0) \
+  X(2048, 2, 512, 1, 3, 1, 2, 4, 0) \
+  X(4096, 2, 1024, 1, 3, 4, 1, 1, 0) \
+  X(4096, 2, 512, 1, 3, 1, 1, 1, 0) \
+  X(4096, 2, 512, 1, 4, 4, 1, 1, 0) \
+  X(4096, 2, 512, 1, 4, 8, 1, 1, 0) \
+  X(4096, 2, 512, 1, 3, 4, 1, 1, 0) \
+  X(4096, 1, 256, 1, 3, 1, 1, 1, 0) \
+  X(4096, 1, 256, 2, 3, 1, 1, 1, 0) \
+  X(4096, 2, 512, 2, 3, 1, 1, 1, 0) \
+  X(4096, 2, 1024, 1, 3, 1, 1, 1, 0) \
+  X(4096, 2, 1024, 1, 3, 8, 1, 1, 0) \
+  X(4096, 1, 512, 1, 3, 1, 1, 1, 0) \
+  X(4096, 1, 512, 2, 3, 1, 1, 1, 0) \
+  X(4096, 2, 512, 1, 3, 1, 0, 1, 0)
+
+struct KVariant {
+  int L, C, T, KS, PS, OFF, PK, W, PF;
+  void (*kern)(KPassArgs);
+  void (*batched)(const KPassArgs*, int);
+  // work items per column group count: the slots cover nyf columns (PK: nyf-1)
+  int groups(int nyf) const { return ((PK ? nyf - 1 : nyf) + C - 1) / C; }
+};
+
+// rpass variants {NY, TLR, PAIRS, W, PFR}: TLR threads per row pair (E = NY/TLR
+// points per thread), PAIRS row pairs per workgroup, W minimum waves per SIMD,
+// PFR = 1 loads both rows' G fields up front (shapes with E = 4), PFR = 2 runs
+// one row per TLR threads instead of a pair; the first listed for each NY is
+// the default.  Single rows win at 256 to 1024 (1024: 20.5 against 21.8 us per
+// launch, 512: 10.8 against 13.5 us; these grids run in one round and are
+// bound by a workgroup's chain of transforms), tie at 2048 and lose at 4096
+// (224 against 205 us: four rounds, where the extra transform counts).
+#define NS2D_RPASS_VARIANTS(X) \
+  X(8, 1, 64, 1, 0) \
+  X(16, 1, 64, 1, 0) \
+  X(32, 2, 32, 1, 0) \
+  X(32, 4, 16, 1, 0) \
+  X(64, 4, 16, 1, 0) \
+  X(64, 8, 8, 1, 0) \
+  X(128, 8, 8, 1, 0) \
+  X(128, 16, 4, 1, 0) \
+  X(128, 32, 2, 1, 0) \
+  X(256, 64, 1, 1, 2) \
+  X(256, 64, 1, 1, 0) \
+  X(256, 64, 1, 1, 1) \
+  X(256, 16, 4, 1, 0) \
+  X(256, 32, 2, 1, 0) \
+  X(512, 64, 1, 1, 2) \
+  X(512, 64, 1, 1, 0) \
+  X(512, 32, 2, 1, 0) \
+  X(512, 128, 1, 1, 0) \
+  X(512, 128, 1, 1, 1) \
+  X(512, 64, 2, 1, 0) \
+  X(512, 64, 4, 1, 0) \
+  X(512, 128, 1, 1, 2) \
+  X(1024, 256, 1, 1, 2) \
+  X(1024, 256, 1, 1, 0) \
+  X(1024, 256, 1, 1, 1) \
+  X(1024, 64, 1, 1, 0) \
+  X(1024, 128, 1, 1, 0) \
+  X(1024, 64, 2, 1, 0) \
+  X(1024, 64, 4, 1, 0) \
+  X(1024, 128, 1, 1, 2) \
+  X(2048, 128, 1, 2, 0) \
+  X(2048, 128, 1, 1, 0) \
+  X(2048, 256, 1, 1, 0) \
+  X(2048, 512, 1, 1, 0) \
+  X(2048, 512, 1, 1, 1) \
+  X(2048, 512, 1, 1, 2) \
+  X(4096, 512, 1, 4, 0) \
+  X(4096, 256, 1, 1, 0) \
+  X(4096, 256, 1, 2, 0) \
+  X(4096, 512, 1, 1, 0) \
+  X(4096, 1024, 1, 1, 0) \
+  X(4096, 1024, 1, 1, 1) \
+  X(4096, 512, 1, 4, 2) \
+  X(4096, 1024, 1, 1, 2)
+
+struct RVariant {
+  int NY, TLR, PAIRS, W, PFR;
+  void (*kern)(RPassArgs);
+  void (*batched)(const RPassArgs*, int);
+};
+
+// The shape tables, assembled from the per-length translation units in
+// list order (so a shape's index among those of its length is its position
+// in the lists above).
+const std::vector<KVariant>& kvariant_table();
+const std::vector<RVariant>& rvariant_table();
+
+}  // namespace ns2d
