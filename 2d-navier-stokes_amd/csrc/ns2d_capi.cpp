@@ -567,6 +567,19 @@ static int create_impl(const ns2d_params* p, hipStream_t ext_stream, ns2d_t** ou
       if (v >= 0 && v < rpass_variant_count(p->ny)) h->rvar = v;
     }
   }
+  // coefficient tables of every viscous kind the handle can reach (the
+  // reference's flag assignment, solver.c:843, switches the kind once), so
+  // that stepping and measuring allocate nothing after creation
+  {
+    const int hyp[2] = {p->hyper_flag, 1};
+    const bool may_switch = (p->quirks & NS2D_QUIRK_VISC_ASSIGN) && p->drag_flag != 1;
+    for (int i = 0; i < (may_switch ? 2 : 1); ++i) {
+      h->hyper_eff = hyp[i];
+      if (!lam_table(h, visc_kind_plain(h)) || !lam_table(h, visc_kind_rk(h)))
+        return cleanup(fail(NS2D_ERR_HIP, "coefficient table allocation failed"));
+    }
+    h->hyper_eff = p->hyper_flag;
+  }
   int rc = reset_stat(h);
   if (rc) return cleanup(rc);
   *out = h;
@@ -764,33 +777,84 @@ int ns2d_rhs(ns2d_t* h, const double* in, double* out) {
   return NS2D_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Measures kernel arguments for the handle's state; with_flux: the RHS is in kout.
+MeasArgs measures_args(ns2d_handle* h, int with_flux, const double* lam, double* out) {
+  MeasArgs a{};
+  a.w_hat = h->w[h->wcur];
+  a.rhs = with_flux ? h->kout : nullptr;
+  a.out = out;
+  a.nx = h->p.nx;
+  a.ny = h->p.ny;
+  a.ld = h->ld;
+  a.lam = lam;
+  a.quirk_enst_diss = (h->p.quirks & NS2D_QUIRK_ENST_DISS) ? 1 : 0;
+  a.sbst_lo = 0.0;                   // LWR_SBST_LIM^2, data_types.h:175
+  a.sbst_hi = 64.0 * 64.0;           // UPR_SBST_LIM^2, data_types.h:174
+  a.ph = h->ph;
+  return a;
+}
+
+// 4 pi^2 * 0.5 / (Nx Ny)^2 (sys_msr.c:43-44)
+double measures_cn(const ns2d_handle* h) {
+  const double norm_fac = 0.5 / std::pow((double)((long)h->p.nx * h->p.ny), 2.0);
+  const double const_fac = 4.0 * std::pow(M_PI, 2.0);
+  return const_fac * norm_fac;
+}
+
+// The reported totals from the bin-order sums tot[NQ] (sys_msr.c:340-372);
+// wf is the state at the forced mode.
+void measures_from_totals(const ns2d_handle* h, const double* tot, double2 wf, int with_flux, ns2d_measures_t* m) {
+  const double cn = measures_cn(h);
+  std::memset(m, 0, sizeof *m);
+  m->energy = tot[Q_ENRG] * cn;
+  m->enstrophy = tot[Q_ENST] * cn;
+  m->palinstrophy = tot[Q_PALIN] * cn;
+  m->divergence = tot[Q_DIV] * cn;
+  m->enrg_diss = tot[Q_ENRG_DISS] * 2.0 * cn;
+  m->enst_diss = tot[Q_ENST_DISS] * 2.0 * cn;
+  if (h->ph.force_on) m->forcing = std::hypot(h->ph.force_val * wf.x, h->ph.force_val * wf.y) * cn;
+  if (with_flux) {
+    // sys_msr.c:200-225, normalised at :363-372 (the d/dt terms are not)
+    m->d_enrg_dt_sbst = tot[Q_SB_DERIV_K2] + tot[Q_SB_DISS_K2];
+    m->enrg_flux_sbst = tot[Q_SB_DERIV_K2] * cn;
+    m->enrg_diss_sbst = tot[Q_SB_DISS_K2] * 2.0 * cn;
+    m->d_enst_dt_sbst = tot[Q_SB_DERIV] + tot[Q_SB_DISS];
+    m->enst_flux_sbst = tot[Q_SB_DERIV] * cn;
+    m->enst_diss_sbst = tot[Q_SB_DISS] * 2.0 * cn;
+  }
+}
+
+// sys_msr.c:125-133: NonlinearRHSBatch(w_hat) (with its ForceConjugacy write
+// into w_hat) into kout; the forcing is subtracted inside the reduction
+int measures_flux_rhs(ns2d_handle* h) {
+  int rc = prepare_inv(h);
+  if (rc) return rc;
+  rc = finish_rhs_into_kout(h, false);
+  if (rc) return rc;
+  if (h->p.quirks & NS2D_QUIRK_CONJ_GLOBAL) h->inv_valid = false;
+  return NS2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int ns2d_measures(ns2d_t* h, int with_flux, ns2d_measures_t* m, double* enrg_spec, double* enst_spec,
                   double* flux_spec) {
   if (!h) return fail(NS2D_ERR_INVALID, "null handle");
   HIPCHK(hipSetDevice(h->p.device));
   int rc;
   if (with_flux) {
-    // sys_msr.c:125-133: NonlinearRHSBatch(w_hat) (with its ForceConjugacy
-    // write into w_hat), minus the forcing
-    rc = prepare_inv(h);
+    rc = measures_flux_rhs(h);
     if (rc) return rc;
-    rc = finish_rhs_into_kout(h, false);
-    if (rc) return rc;
-    if (h->p.quirks & NS2D_QUIRK_CONJ_GLOBAL) h->inv_valid = false;
   }
-  MeasArgs a{};
-  a.w_hat = h->w[h->wcur];
-  a.rhs = with_flux ? h->kout : nullptr;
-  a.out = h->meas;
-  a.nx = h->p.nx;
-  a.ny = h->p.ny;
-  a.ld = h->ld;
-  a.lam = lam_table(h, visc_kind_plain(h));  // sys_msr.c:160-175
-  if (!a.lam) return fail(NS2D_ERR_HIP, "coefficient table allocation failed");
-  a.quirk_enst_diss = (h->p.quirks & NS2D_QUIRK_ENST_DISS) ? 1 : 0;
-  a.sbst_lo = 0.0;                   // LWR_SBST_LIM^2, data_types.h:175
-  a.sbst_hi = 64.0 * 64.0;           // UPR_SBST_LIM^2, data_types.h:174
-  a.ph = h->ph;
+  const double* lam = lam_table(h, visc_kind_plain(h));  // sys_msr.c:160-175
+  if (!lam) return fail(NS2D_ERR_HIP, "coefficient table allocation failed");
+  const MeasArgs a = measures_args(h, with_flux, lam, h->meas);
   hipError_t e = launch_measures(h->nspect, a, h->stream);
   if (e != hipSuccess) return fail(NS2D_ERR_HIP, "measures launch: %s", hipGetErrorString(e));
   std::vector<double> part((size_t)NQ * h->nspect);
@@ -799,32 +863,11 @@ int ns2d_measures(ns2d_t* h, int with_flux, ns2d_measures_t* m, double* enrg_spe
   if (h->ph.force_on)
     HIPCHK(hipMemcpyAsync(&wf, h->w[h->wcur] + h->p.force_k, sizeof wf, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
-  const double norm_fac = 0.5 / std::pow((double)((long)h->p.nx * h->p.ny), 2.0);  // sys_msr.c:43
-  const double const_fac = 4.0 * std::pow(M_PI, 2.0);
-  const double cn = const_fac * norm_fac;
+  const double cn = measures_cn(h);
   double tot[NQ] = {0};
   for (int b = 0; b < h->nspect; ++b)
     for (int q = 0; q < NQ; ++q) tot[q] += part[(size_t)b * NQ + q];
-  // with_flux: the forcing is subtracted from the RHS inside the reduction
-  if (m) {
-    std::memset(m, 0, sizeof *m);
-    m->energy = tot[Q_ENRG] * cn;
-    m->enstrophy = tot[Q_ENST] * cn;
-    m->palinstrophy = tot[Q_PALIN] * cn;
-    m->divergence = tot[Q_DIV] * cn;
-    m->enrg_diss = tot[Q_ENRG_DISS] * 2.0 * cn;
-    m->enst_diss = tot[Q_ENST_DISS] * 2.0 * cn;
-    if (h->ph.force_on) m->forcing = std::hypot(h->ph.force_val * wf.x, h->ph.force_val * wf.y) * cn;
-    if (with_flux) {
-      // sys_msr.c:200-225, normalised at :363-372 (the d/dt terms are not)
-      m->d_enrg_dt_sbst = tot[Q_SB_DERIV_K2] + tot[Q_SB_DISS_K2];
-      m->enrg_flux_sbst = tot[Q_SB_DERIV_K2] * cn;
-      m->enrg_diss_sbst = tot[Q_SB_DISS_K2] * 2.0 * cn;
-      m->d_enst_dt_sbst = tot[Q_SB_DERIV] + tot[Q_SB_DISS];
-      m->enst_flux_sbst = tot[Q_SB_DERIV] * cn;
-      m->enst_diss_sbst = tot[Q_SB_DISS] * 2.0 * cn;
-    }
-  }
+  if (m) measures_from_totals(h, tot, wf, with_flux, m);
   for (int b = 0; b < h->nspect; ++b) {
     const double* q = &part[(size_t)b * NQ];
     if (enrg_spec) enrg_spec[b] = q[Q_ENRG] * cn;

@@ -60,6 +60,46 @@ def energy(w_hat: np.ndarray, Nx: int, Ny: int) -> float:
     return float(s) * const_fac * norm_fac
 
 
+def palinstrophy(w_hat: np.ndarray, Nx: int, Ny: int) -> float:
+    """Total palinstrophy as ComputeSystemMeasurables forms it (sys_msr.c:195,223,362)."""
+    kx, ky = wavenumbers(Nx, Ny)
+    k2 = (kx * kx + ky * ky).astype(np.float64)
+    c = np.full((1, Ny // 2 + 1), 2.0)
+    c[0, 0] = c[0, -1] = 1.0
+    a2 = w_hat.real ** 2 + w_hat.imag ** 2
+    s = np.sum(c * k2 * a2)
+    return float(s) * 4.0 * math.pi ** 2 * (0.5 / float(Nx * Ny) ** 2)
+
+
+def random_phase_ep(Nx: int, Ny: int, seed: int, k0: float = 6.0, e0: float = 0.5, p0: float = 5000.0,
+                    filt: str = "23") -> np.ndarray:
+    """random_phase's k^4 exp(-(k/k0)^2) field projected onto fixed energy e0
+    and palinstrophy p0 (the initial conditions of extreme-enstrophy searches):
+    w_hat -> a (|k|/k0)^b w_hat with b found by bisection so that P/E = p0/e0
+    (the ratio grows monotonically with b), then a so that E = e0."""
+    base = random_phase(Nx, Ny, seed, k0=k0, e0=e0, filt=filt)
+    kx, ky = wavenumbers(Nx, Ny)
+    q = np.sqrt((kx * kx + ky * ky).astype(np.float64)) / k0
+    q[0, 0] = 1.0
+
+    def ratio(b):
+        w = base * q ** b
+        return palinstrophy(w, Nx, Ny) / energy(w, Nx, Ny)
+
+    target = p0 / e0
+    lo, hi = -4.0, 4.0
+    if not (ratio(lo) < target < ratio(hi)):
+        raise ValueError(f"palinstrophy {p0} not reachable at energy {e0} for k0={k0}")
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if ratio(mid) < target:
+            lo = mid
+        else:
+            hi = mid
+    w = base * q ** (0.5 * (lo + hi))
+    return w * math.sqrt(e0 / energy(w, Nx, Ny))
+
+
 def random_phase(Nx: int, Ny: int, seed: int, k0: float = 8.0, e0: float = 0.5, filt: str = "23",
                  spectrum: str = "k4exp", kmin: float = 0.0, kmax: float | None = None) -> np.ndarray:
     """Seeded random-phase vorticity: |w_hat|^2 = |k| E(|k|) with E(k) ~ k^4 exp(-(k/k0)^2)
@@ -90,6 +130,21 @@ def random_phase(Nx: int, Ny: int, seed: int, k0: float = 8.0, e0: float = 0.5, 
     if e > 0:
         w_hat *= math.sqrt(e0 / e)
     return w_hat
+
+
+def cfl_timestep(w_hat: np.ndarray, Nx: int, Ny: int, nu: float, cfl: float = 0.5) -> float:
+    """GetTimestep's CFL formula (solver.c:2381-2450) on the host, with every
+    mode's velocity (the reference's u_hat indexing at solver.c:2394 offsets
+    the rows): cfl / (pi (max|u|/dx + max|v|/dy) + pi^2 nu (1/dx^2 + 1/dy^2)),
+    u = c2r(i ky w_hat/|k|^2)/(Nx Ny), v = c2r(-i kx w_hat/|k|^2)/(Nx Ny)."""
+    kx, ky = wavenumbers(Nx, Ny)
+    k2 = (kx * kx + ky * ky).astype(np.float64)
+    inv = np.where(k2 > 0, 1.0 / np.where(k2 > 0, k2, 1.0), 0.0)
+    u = np.fft.irfft2(1j * ky * inv * w_hat, s=(Nx, Ny), norm="forward") / (Nx * Ny)
+    v = np.fft.irfft2(-1j * kx * inv * w_hat, s=(Nx, Ny), norm="forward") / (Nx * Ny)
+    dx, dy = 2 * math.pi / Nx, 2 * math.pi / Ny
+    conv = float(np.max(math.pi * (np.abs(u) / dx + np.abs(v) / dy)))
+    return cfl / (conv + math.pi ** 2 * nu * (1 / dx ** 2 + 1 / dy ** 2))
 
 
 def taylor_green(solver, kappa: float = 1.0, filt: str = "23") -> np.ndarray:

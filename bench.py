@@ -80,17 +80,9 @@ def time_gpu(nx, ny, steps, warmup):
 
 
 def cfl_dt(w_hat, nx, ny, nu, cfl=0.5):
-    """GetTimestep's CFL formula (solver.c:2381-2450) on the host with every
-    mode's velocity: cfl / (pi (max|u|/dx + max|v|/dy) + pi^2 nu (1/dx^2 + 1/dy^2))."""
+    """GetTimestep's CFL step (solver.c:2381-2450) with every mode's velocity."""
     from ns2d import ics
-    kx, ky = ics.wavenumbers(nx, ny)
-    k2 = (kx * kx + ky * ky).astype(np.float64)
-    inv = np.where(k2 > 0, 1.0 / np.where(k2 > 0, k2, 1.0), 0.0)
-    u = np.fft.irfft2(1j * ky * inv * w_hat, s=(nx, ny), norm="forward") / (nx * ny)
-    v = np.fft.irfft2(-1j * kx * inv * w_hat, s=(nx, ny), norm="forward") / (nx * ny)
-    dx, dy = 2 * math.pi / nx, 2 * math.pi / ny
-    conv = np.max(math.pi * (np.abs(u) / dx + np.abs(v) / dy))
-    return cfl / (conv + math.pi ** 2 * nu * (1 / dx ** 2 + 1 / dy ** 2))
+    return ics.cfl_timestep(w_hat, nx, ny, nu, cfl)
 
 
 def ensemble_leg(steps=500, seed=2024):
