@@ -16,8 +16,9 @@ relative to a forward solve (512^2, 2000 steps).
 The roofline object covers the dominant kernel, timed live with HIP events
 recorded on the solver's own stream around every launch; its algorithmic
 bytes per launch are stated in alg_bytes() below.  The CPU baseline is the
-numpy/scipy restatement of the reference algorithm (oracle/) on the host
-cores, timed on a bounded number of steps of the same workload.
+compiled C restatement of the reference's RK4CN step (oracle/ns2d_cpu.c,
+FFTW3 interface of MKL, OpenMP on the host cores), timed on a bounded number
+of steps of the same workload (median of 3).
 """
 from __future__ import annotations
 
@@ -193,26 +194,65 @@ def kolmogorov_leg(n=4096, steps=40, warmup=5, seed=3):
             "kolm_ab4_4096_model272_frac": 272.0 * M * sps / 1e9 / HBM_PEAK_GBS}
 
 
+def _host_threads():
+    """Threads the host run may use: OMP_NUM_THREADS when set (the GPU box
+    sets it to the box's CPU share), else the CPUs this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(nx, ny, budget_s=20.0):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import ns2d_oracle as O
-    cores = max(1, min(16, os.cpu_count() or 1))
-    O.set_fft_workers(cores)
-    p = O.Params(nx, ny, nu=NU, hyper_flag=0, hyper_pow=1.0, filter="houli_ref")
-    st = O.State(make_ic(nx, ny), p)
-    O.step(st, DT, 1)  # warm-up (plans, page faults)
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        O.step(st, DT, 1)
-        O.max_vort(st.w_hat, p)   # the reference's per-step SystemCheck
-        n += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or n >= 200:
-            break
-    return {"value": n / el, "unit": "steps/s", "cores": cores, "kind": "port",
-            "sample": f"{n} RK4CN steps of the {nx}x{ny} workload incl. per-step max|w| "
-                      f"(numpy/scipy.fft restatement, {cores} FFT threads), {el:.1f} s"}
+    """The compiled CPU restatement (oracle/ns2d_cpu: the reference's RK4CN
+    step in C, FFTW3 interface of MKL, OpenMP) on the host cores: steps/s of
+    the same workload from the same initial condition, CLOCK_MONOTONIC around
+    the stepping loop, median of 3 repetitions, with the per-step max|w|
+    check (the reference's SystemCheck) and, separately, without it.  The step
+    count per repetition is sized from a short calibration run so that the
+    whole leg takes about budget_s."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "ns2d_cpu")
+    threads = _host_threads()
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} not built: run __graft_entry__.build()")
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), MKL_THREADING_LAYER="GNU")
+    with tempfile.TemporaryDirectory() as tmp:
+        icp = os.path.join(tmp, "ic.bin")
+        make_ic(nx, ny).astype(np.complex128).tofile(icp)
+
+        def run(steps, check, reps):
+            r = subprocess.run([exe, str(nx), str(ny), repr(NU), "1.0", "houli_ref", repr(DT), str(steps),
+                                str(check), str(reps), icp], capture_output=True, text=True, env=env, check=True)
+            return json.loads(r.stdout.strip().splitlines()[-1])
+
+        cal = run(2, 1, 1)
+        per_step = 1.0 / cal["median_steps_per_s"]
+        steps = max(2, int(budget_s / 6.0 / per_step))
+        with_chk = run(steps, 1, 3)
+        no_chk = run(steps, 0, 3)
+    return {"value": with_chk["median_steps_per_s"], "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} RK4CN steps of the {nx}x{ny} workload per repetition, median of 3, incl. the "
+                      f"per-step max|w| check (compiled C restatement, FFTW3 interface of MKL, "
+                      f"{threads} OpenMP threads)",
+            "reps_steps_per_s": with_chk["steps_per_s"],
+            "without_check_steps_per_s": no_chk["median_steps_per_s"],
+            "cpu_model": _cpu_model()}
 
 
 def main():
@@ -264,7 +304,7 @@ def main():
         extra.update(kolmogorov_leg())
         if not args.no_cpu:
             # the same RK4CN workload at 4096^2 on the host cores, a few steps
-            c4 = cpu_baseline(4096, 4096, budget_s=12.0)
+            c4 = cpu_baseline(4096, 4096, budget_s=30.0)
             extra.update({"cpu_4096": c4, "rk4_4096_vs_cpu": s4 / c4["value"]})
     if not args.no_ensemble:
         extra.update(ensemble_leg())
