@@ -390,6 +390,27 @@ def max_vort(w_hat: np.ndarray, p: Params) -> float:
     return float(np.max(np.abs(real_vorticity(w_hat, p))))
 
 
+def cfl_timestep(w_hat: np.ndarray, p: Params, cfl: float) -> float:
+    """GetTimestep's CFL branch, solver.c:2381-2450: u_hat = i ky/|k|^2 w_hat,
+    v_hat = -i kx/|k|^2 w_hat (:2392-2401), u, v = c2r(.)/(Nx Ny) (:2409-2420),
+    dt = cfl / (pi (max|u|/dx + max|v|/dy) + pi^2 nu (1/dx^2 + 1/dy^2))
+    (:2425-2447), max taken over the sum per point as the reference's loop
+    does.  The reference's u_hat index at :2394 offsets the rows (its u is
+    wrong); this uses every mode's own velocity, as the driver does."""
+    Nx, Ny = p.Nx, p.Ny
+    kx, ky = wavenumbers(Nx, Ny)
+    k2 = (kx * kx + ky * ky).astype(np.float64)
+    k2s = np.where(k2 > 0, k2, 1.0)
+    u_hat = np.where(k2 > 0, 1j * ky * w_hat / k2s, 0.0)
+    v_hat = np.where(k2 > 0, -1j * kx * w_hat / k2s, 0.0)
+    u = c2r(u_hat, Nx, Ny) * (1.0 / (Nx * Ny))
+    v = c2r(v_hat, Nx, Ny) * (1.0 / (Nx * Ny))
+    dx, dy = 2.0 * math.pi / Nx, 2.0 * math.pi / Ny
+    conv = float(np.max(math.pi * (np.abs(u) / dx + np.abs(v) / dy)))
+    diff = math.pi ** 2 * (p.nu / dx ** 2 + p.nu / dy ** 2)
+    return cfl / (conv + diff)
+
+
 def n_spect(Nx: int, Ny: int) -> int:
     """sys_msr.c:394."""
     return int(round(math.sqrt((Nx / 2.0) * (Nx / 2.0) + (Ny / 2.0) * (Ny / 2.0)) + 1))

@@ -153,3 +153,42 @@ def test_blow_up_stops_the_run(tmp_path):
 def test_bad_command_line_is_rejected(tmp_path):
     r = run(["-n", "63"], cwd=str(tmp_path), expect_rc=1)
     assert "must be a multiple of 2" in r.stderr
+
+
+def test_adaptive_cfl_step_matches_oracle(tmp_path):
+    """-h dt -h 1 with the CFL condition (-c 1 -c 0.5): every step's size is
+    GetTimestep's CFL formula on the current state (solver.c:2381-2450, with
+    every mode's velocity), the initial one on the initial condition
+    (InitializeIntegrationVariables, solver.c:2116-2166).  The TimeStep and
+    TimeValue attributes and w_hat of every save against the oracle run with
+    the same formula."""
+    Nx = Ny = 64
+    nu, T, every, cfl = 5e-3, 0.03, 4, 0.5
+    run(["-o", str(tmp_path) + "/", "-n", str(Nx), "-n", str(Ny), "-s", "0.0", "-e", str(T), "-h", "0.001",
+         "-h", "1", "-c", "1", "-c", str(cfl), "-v", str(nu), "-v", "0", "-v", "1.0", "-i", "DECAY_TURB_BB",
+         "-p", str(every)], cwd=str(tmp_path))
+    d = out_dir(tmp_path)
+    p = O.Params(Nx, Ny, nu=nu, hyper_flag=0, hyper_pow=1.0)
+    st = O.State(R.initial_condition("DECAY_TURB_BB", Nx, Ny), p)
+    dt = O.cfl_timestep(st.w_hat, p, cfl)
+    t, iters, saves = dt, 1, [(0.0, dt, st.w_hat.copy())]
+    while t <= T:
+        O.step(st, dt, 1)
+        if iters % every == 0:
+            O.force_conjugacy(st.w_hat, st.w_hat)   # the save's flux RHS (sys_msr.c:126)
+            saves.append((t, dt, st.w_hat.copy()))
+        iters += 1
+        dt = O.cfl_timestep(st.w_hat, p, cfl)
+        t += dt
+    assert len(saves) >= 4
+    with File(os.path.join(d, "Main_HDF_Data.h5")) as h:
+        assert h.keys() == [f"Iter_{k:05d}" for k in range(len(saves))]
+        steps = set()
+        for k, (tv, dtv, w) in enumerate(saves):
+            g = f"/Iter_{k:05d}"
+            np.testing.assert_allclose(h.attr(g, "TimeValue"), tv, rtol=1e-12)
+            np.testing.assert_allclose(h.attr(g, "TimeStep"), dtv, rtol=1e-12)
+            steps.add(round(float(h.attr(g, "TimeStep")), 12))
+            got = h.read(g + "/w_hat")
+            assert np.linalg.norm(got - w) <= 1e-11 * np.linalg.norm(w), k
+        assert len(steps) == len(saves)     # the step size changes from save to save

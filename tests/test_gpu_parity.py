@@ -251,3 +251,47 @@ def test_pass_variants_match_oracle(Nx, Ny, var, rvar, integ):
     # the row pass's vorticity transform (max|w| check, c2r)
     assert abs(mv - O.max_vort(st.w_hat, p)) <= 1e-12 * abs(mv)
     assert rel(wr, O.c2r(w0, Nx, Ny)) < 1e-13
+
+
+def test_flux_terms_are_inviscid_rates_of_change_on_gpu():
+    """The GPU's flux subsets and flux spectra (ns2d_measures with_flux) at
+    256^2, nu = 0: d/dt of the subset enstrophy/energy and of the spectra
+    along the GPU's own RK4CN trajectory (Richardson-combined forward
+    differences), to 1e-6 relative."""
+    p = O.Params(256, 256, nu=0.0, hyper_flag=0, hyper_pow=1.0, filter="23", quirks=0)
+    w0 = O.ic_random_spectrum(p, seed=21, k0=30.0)
+    h = 2e-6
+    kx, ky = O.wavenumbers(256, 256)
+    k2 = (kx * kx + ky * ky).astype(np.float64)
+    c = np.full((1, 129), 2.0)
+    c[0, 0] = c[0, -1] = 1.0
+    sb = (k2 > 0) & (k2 < 64.0 ** 2)
+    C = 4.0 * np.pi ** 2 * 0.5 / float(256 * 256) ** 2
+
+    def sums(w, m):
+        a2 = np.abs(w) ** 2
+        return np.array([np.sum(np.where(sb, c * a2, 0.0)) * C,
+                         np.sum(np.where(sb, c * a2 / np.where(k2 > 0, k2, 1.0), 0.0)) * C]), m
+
+    with solver(p, check=0) as s:
+        s.set_state(w0)
+        m0 = s.system_measures(with_flux=True)
+        base = sums(w0, m0)
+
+        def rate(dt):
+            s.set_state(w0)
+            s.spectral_step(dt, 1)
+            w = s.get_state()
+            m = s.system_measures()
+            x = sums(w, m)
+            return ((x[0] - base[0]) / dt, (m["enst_spect"] - m0["enst_spect"]) / dt,
+                    (m["enrg_spect"] - m0["enrg_spect"]) / dt)
+
+        a, b = rate(h / 2), rate(h)
+        sub = 2 * a[0] - b[0]
+        zs = 2 * a[1] - b[1]
+        es = 2 * a[2] - b[2]
+    assert abs(sub[0] - m0["enst_flux_sbst"]) <= 1e-6 * abs(sub[0])
+    assert abs(sub[1] - m0["enrg_flux_sbst"]) <= 1e-6 * abs(sub[1])
+    assert np.linalg.norm(zs - m0["enst_flux_spect"]) <= 1e-6 * np.linalg.norm(zs)
+    assert np.linalg.norm(es - m0["enrg_flux_spect"]) <= 1e-6 * np.linalg.norm(es)

@@ -258,3 +258,111 @@ def test_kolmogorov_forcing_value():
     w = np.zeros((16, 9), dtype=complex)
     K = O.nonlinear_rhs(w, p)
     assert K[0, 4] == -2.0 and np.count_nonzero(K) == 1
+
+
+# ---------------------------------------------------------------------------
+# flux diagnostics (sys_msr.c:125-133, 195-305, 363-372) and the CFL step
+# (solver.c:2381-2450), pinned by what the dynamics fixes
+# ---------------------------------------------------------------------------
+def _subset_sums(w, p, lim=64.0):
+    """Z and E restricted to the flux subset 0 <= |k|^2 < lim^2 (data_types.h:174-175),
+    with the half-spectrum weights and normalisation of the totals."""
+    kx, ky = O.wavenumbers(p.Nx, p.Ny)
+    k2 = (kx * kx + ky * ky).astype(np.float64)
+    c = np.full((1, p.Ny // 2 + 1), 2.0)
+    c[0, 0] = c[0, -1] = 1.0
+    sb = (k2 > 0) & (k2 < lim * lim)
+    a2 = np.abs(w) ** 2
+    C = 4.0 * np.pi ** 2 * 0.5 / float(p.Nx * p.Ny) ** 2
+    return (float(np.sum(np.where(sb, c * a2, 0.0))) * C,
+            float(np.sum(np.where(sb, c * a2 / np.where(k2 > 0, k2, 1.0), 0.0))) * C)
+
+
+def _richardson_rate(f, w0, p, h):
+    """d f(w(t))/dt at t = 0 along the inviscid RK4CN trajectory: two forward
+    differences (steps h and h/2), Richardson-combined (error O(h^2))."""
+    def D(dt):
+        st = O.State(w0, p)
+        O.step(st, dt, 1)
+        return (f(st.w_hat) - f(w0)) / dt
+    return 2.0 * D(h / 2) - D(h)
+
+
+def test_flux_subset_and_spectra_are_the_inviscid_rates_of_change():
+    """At 256^2 the 2/3 disk (|k| < 85) extends past the flux subset's
+    UPR_SBST_LIM = 64, so energy and enstrophy flow across its edge.  With
+    nu = 0 and no forcing, dw/dt = K = NonlinearRHSBatch(w), so the subset
+    fluxes are d/dt of the subset enstrophy/energy and the flux spectra are
+    d/dt of the enstrophy/energy spectra; the time derivatives come from the
+    oracle's own time stepping."""
+    p = O.Params(256, 256, nu=0.0, hyper_flag=0, hyper_pow=1.0, filter="23", quirks=0)
+    w0 = O.ic_random_spectrum(p, seed=21, k0=30.0)
+    K = O.nonlinear_rhs(w0, p)
+    m = O.measures(w0, p, rhs=K)
+    h = 2e-6
+    z_rate = _richardson_rate(lambda w: _subset_sums(w, p)[0], w0, p, h)
+    e_rate = _richardson_rate(lambda w: _subset_sums(w, p)[1], w0, p, h)
+    assert abs(m["enst_flux_sbst"]) > 1e-3 * abs(z_rate) and abs(z_rate) > 0
+    assert abs(z_rate - m["enst_flux_sbst"]) <= 1e-6 * abs(z_rate)
+    assert abs(e_rate - m["enrg_flux_sbst"]) <= 1e-6 * abs(e_rate)
+    zs = _richardson_rate(lambda w: O.measures(w, p)["enst_spect"], w0, p, h)
+    es = _richardson_rate(lambda w: O.measures(w, p)["enrg_spect"], w0, p, h)
+    assert np.linalg.norm(zs - m["enst_flux_spect"]) <= 1e-6 * np.linalg.norm(zs)
+    assert np.linalg.norm(es - m["enrg_flux_spect"]) <= 1e-6 * np.linalg.norm(es)
+    # without viscosity the d/dt series are the unnormalised flux sums
+    C = 4.0 * np.pi ** 2 * 0.5 / float(p.Nx * p.Ny) ** 2
+    assert abs(m["d_enst_dt_sbst"] * C - m["enst_flux_sbst"]) <= 1e-12 * abs(m["enst_flux_sbst"])
+    assert abs(m["d_enrg_dt_sbst"] * C - m["enrg_flux_sbst"]) <= 1e-12 * abs(m["enrg_flux_sbst"])
+
+
+def test_flux_vanishes_on_the_full_disk_and_dissipation_subsets():
+    """At 64^2 the 2/3 disk (|k| < 21) lies inside the subset, so the subset
+    fluxes are the total fluxes, zero by the Galerkin conservation of energy
+    and enstrophy; every flux spectrum sums to zero.  The dissipation subsets
+    are then 2 nu P and 2 nu Z (enst_diss with Q7 off, enrg_diss), and the
+    d/dt series are flux + dissipation in the reference's normalisation."""
+    p = O.Params(64, 64, nu=3e-3, hyper_flag=0, hyper_pow=1.0, filter="23", quirks=0)
+    w0 = O.ic_random_spectrum(p, seed=4, k0=8.0)
+    K = O.nonlinear_rhs(w0, p)
+    m = O.measures(w0, p, rhs=K)
+    kx, ky = O.wavenumbers(p.Nx, p.Ny)
+    k2 = (kx * kx + ky * ky).astype(np.float64)
+    C = 4.0 * np.pi ** 2 * 0.5 / float(p.Nx * p.Ny) ** 2
+    scale_z = 2.0 * C * float(np.sum(np.abs(w0) * np.abs(K)))
+    scale_e = 2.0 * C * float(np.sum(np.abs(w0) * np.abs(K) / np.where(k2 > 0, k2, 1.0)))
+    assert abs(m["enst_flux_sbst"]) <= 1e-13 * scale_z
+    assert abs(m["enrg_flux_sbst"]) <= 1e-13 * scale_e
+    assert abs(np.sum(m["enst_flux_spect"])) <= 1e-13 * scale_z
+    assert abs(np.sum(m["enrg_flux_spect"])) <= 1e-13 * scale_e
+    assert np.max(np.abs(m["enst_flux_spect"])) > 1e-6 * scale_z
+    np.testing.assert_allclose(m["enst_diss_sbst"], 2.0 * p.nu * m["palinstrophy"], rtol=1e-13)
+    np.testing.assert_allclose(m["enrg_diss_sbst"], 2.0 * p.nu * m["enstrophy"], rtol=1e-13)
+    np.testing.assert_allclose(m["enst_diss_sbst"], m["enst_diss"], rtol=1e-13)
+    np.testing.assert_allclose(m["enrg_diss_sbst"], m["enrg_diss"], rtol=1e-13)
+    np.testing.assert_allclose(C * m["d_enst_dt_sbst"], m["enst_flux_sbst"] + 0.5 * m["enst_diss_sbst"],
+                               rtol=1e-12, atol=1e-12 * scale_z)
+    np.testing.assert_allclose(C * m["d_enrg_dt_sbst"], m["enrg_flux_sbst"] + 0.5 * m["enrg_diss_sbst"],
+                               rtol=1e-12, atol=1e-12 * scale_e)
+    np.testing.assert_allclose(m["d_enst_dt_spect"], m["enst_flux_spect"] + m["enst_diss_spect"], rtol=1e-13,
+                               atol=1e-13 * scale_z)
+    np.testing.assert_allclose(m["enst_diss_spect"].sum(), 0.5 * m["enst_diss"], rtol=1e-13)
+
+
+@pytest.mark.parametrize("N,nu", [(64, 0.0), (64, 1e-3), (128, 5e-4)])
+def test_cfl_timestep_closed_form(N, nu):
+    """Taylor-Green w = 2 cos x cos y: psi = cos x cos y, u = -cos x sin y,
+    v = sin x cos y, so max(|u| + |v|) = max |sin(x + y)| or |sin(x - y)| = 1,
+    reached on the grid (x = y = pi/4); with dx = dy = 2 pi/N the CFL step is
+    cfl / (N/2 + pi^2 nu N^2 / (2 pi^2))."""
+    p = O.Params(N, N, nu=nu)
+    x = np.arange(N) * 2 * np.pi / N
+    w = 2.0 * np.cos(x)[:, None] * np.cos(x)[None, :]
+    dt = O.cfl_timestep(O.r2c(w), p, 0.9)
+    dx = 2 * np.pi / N
+    expect = 0.9 / (np.pi * (1.0 / dx) + np.pi ** 2 * nu * 2.0 / dx ** 2)
+    assert abs(dt - expect) <= 1e-14 * expect
+    # velocity of a single Fourier mode: w = cos(3 y) -> u = -sin(3y)/3, v = 0
+    w = np.cos(3 * x)[None, :] * np.ones((N, 1))
+    dt = O.cfl_timestep(O.r2c(w), p, 0.5)
+    expect = 0.5 / (np.pi * (1.0 / 3.0) / dx + np.pi ** 2 * nu * 2.0 / dx ** 2)
+    assert abs(dt - expect) <= 1e-13 * expect
