@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <sys/stat.h>
 #include <vector>
 
 #include "../../include/ns2d_gpu.h"
@@ -66,6 +67,7 @@ struct ns2d_handle {
   double* meas = nullptr;
   double* wreal = nullptr;
   double* rbuf = nullptr;
+  double2* stage = nullptr;  // [Nx][Ny/2+1] staging between the host and device layouts (x-tiles only)
   bool inv_valid = false;  // G1..G4 and conj[cur] hold the transforms of the state
   int cur = 0;             // conj[cur] = ky=0 column of the current stage input
   long iters = 0;
@@ -143,7 +145,7 @@ const double* lam_table(ns2d_handle* h, int kind) {
         case VISC_HYPER: v = p.nu * std::pow(k_sqr, p.hyper_pow); break;
         default: v = p.nu * k_sqr; break;
       }
-      t[(size_t)i * ld + j] = v;
+      t[spec_idx(i, j, ld)] = v;
     }
   }
   double* d = nullptr;
@@ -189,6 +191,30 @@ RPassArgs rbase(ns2d_handle* h, int mode) {
   a.mode = mode;
   a.wscale = 1.0 / (double)((long)h->p.nx * h->p.ny);
   return a;
+}
+
+// A [Nx][Ny/2+1] complex array of a caller (host or device memory, `kind`)
+// into / out of the handle's device layout.
+int copy_in(ns2d_handle* h, double2* dst, const void* src, hipMemcpyKind kind) {
+  const size_t rowb = h->nyf * sizeof(double2);
+  if constexpr (XT == 1) {
+    HIPCHK(hipMemcpy2DAsync(dst, h->ld * sizeof(double2), src, rowb, rowb, h->p.nx, kind, h->stream));
+  } else {
+    HIPCHK(hipMemcpyAsync(h->stage, src, rowb * h->p.nx, kind, h->stream));
+    HIPCHK(launch_relayout(h->stage, dst, h->p.nx, h->nyf, h->ld, 0, h->stream));
+  }
+  return NS2D_OK;
+}
+
+int copy_out(ns2d_handle* h, void* dst, const double2* src, hipMemcpyKind kind) {
+  const size_t rowb = h->nyf * sizeof(double2);
+  if constexpr (XT == 1) {
+    HIPCHK(hipMemcpy2DAsync(dst, rowb, src, h->ld * sizeof(double2), rowb, h->p.nx, kind, h->stream));
+  } else {
+    HIPCHK(launch_relayout(src, h->stage, h->p.nx, h->nyf, h->ld, 1, h->stream));
+    HIPCHK(hipMemcpyAsync(dst, h->stage, rowb * h->p.nx, kind, h->stream));
+  }
+  return NS2D_OK;
 }
 
 enum { KIND_K = 0, KIND_R = 1, KIND_OTHER = 2 };
@@ -520,6 +546,8 @@ static int create_impl(const ns2d_params* p, hipStream_t ext_stream, ns2d_t** ou
   }
   if (hipMalloc((void**)&h->pairbuf, sizeof(double2) * 8 * p->nx) != hipSuccess)
     return cleanup(fail(NS2D_ERR_HIP, "hipMalloc failed"));
+  if (XT > 1 && hipMalloc((void**)&h->stage, sizeof(double2) * p->nx * h->nyf) != hipSuccess)
+    return cleanup(fail(NS2D_ERR_HIP, "hipMalloc failed"));
   if (hipMalloc((void**)&h->stat, sizeof(unsigned long long) * STAT_N) != hipSuccess)
     return cleanup(fail(NS2D_ERR_HIP, "hipMalloc failed"));
   if (hipMalloc((void**)&h->meas, sizeof(double) * NQ * h->nspect) != hipSuccess)
@@ -541,7 +569,7 @@ static int create_impl(const ns2d_params* p, hipStream_t ext_stream, ns2d_t** ou
     for (int i = 0; i < p->nx; ++i) {
       const long kx = i <= p->nx / 2 ? i : i - p->nx;
       for (int j = 0; j < h->nyf; ++j)
-        t[(size_t)i * h->ld + j] = std::exp(-36.0 * std::pow(std::sqrt(std::pow(kx / (p->nx / 2.0), 2.0) +
+        t[spec_idx(i, j, h->ld)] = std::exp(-36.0 * std::pow(std::sqrt(std::pow(kx / (p->nx / 2.0), 2.0) +
                                                                         std::pow(j / (p->ny / 2.0), 2.0)), 36.0));
     }
     if (hipMalloc((void**)&h->filt_tab, sizeof(double) * t.size()) != hipSuccess ||
@@ -608,6 +636,7 @@ int ns2d_destroy(ns2d_t* h) {
   if (h->meas) (void)hipFree(h->meas);
   if (h->wreal) (void)hipFree(h->wreal);
   if (h->rbuf) (void)hipFree(h->rbuf);
+  if (h->stage) (void)hipFree(h->stage);
   drop_graph(h);
   for (auto& g : h->adj_graph)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -622,8 +651,8 @@ int ns2d_destroy(ns2d_t* h) {
 static int set_state_impl(ns2d_t* h, const void* src, hipMemcpyKind kind) {
   if (!h || !src) return fail(NS2D_ERR_INVALID, "null argument");
   HIPCHK(hipSetDevice(h->p.device));
-  HIPCHK(hipMemcpy2DAsync(h->w[h->wcur], h->ld * sizeof(double2), src, h->nyf * sizeof(double2),
-                          h->nyf * sizeof(double2), h->p.nx, kind, h->stream));
+  int rc0 = copy_in(h, h->w[h->wcur], src, kind);
+  if (rc0) return rc0;
   h->inv_valid = false;
   h->iters = 0;
   h->hyper_eff = h->p.hyper_flag;
@@ -633,8 +662,8 @@ static int set_state_impl(ns2d_t* h, const void* src, hipMemcpyKind kind) {
 static int get_state_impl(ns2d_t* h, void* dst, hipMemcpyKind kind) {
   if (!h || !dst) return fail(NS2D_ERR_INVALID, "null argument");
   HIPCHK(hipSetDevice(h->p.device));
-  HIPCHK(hipMemcpy2DAsync(dst, h->nyf * sizeof(double2), h->w[h->wcur], h->ld * sizeof(double2),
-                          h->nyf * sizeof(double2), h->p.nx, kind, h->stream));
+  int rc0 = copy_out(h, dst, h->w[h->wcur], kind);
+  if (rc0) return rc0;
   HIPCHK(hipStreamSynchronize(h->stream));
   return NS2D_OK;
 }
@@ -759,20 +788,20 @@ int ns2d_step(ns2d_t* h, double dt, long nsteps) { return step_impl(h, dt, nstep
 int ns2d_rhs(ns2d_t* h, const double* in, double* out) {
   if (!h || !in || !out) return fail(NS2D_ERR_INVALID, "null argument");
   HIPCHK(hipSetDevice(h->p.device));
-  HIPCHK(hipMemcpy2DAsync(h->sbuf, h->ld * sizeof(double2), in, h->nyf * sizeof(double2), h->nyf * sizeof(double2),
-                          h->p.nx, hipMemcpyHostToDevice, h->stream));
+  int rc = copy_in(h, h->sbuf, in, hipMemcpyHostToDevice);
+  if (rc) return rc;
   KPassArgs a = kbase(h);
   a.mode = KP_INV;
   a.epi = EPI_FROM_INPUT;
   a.sin = h->sbuf;
   a.conj_dst = h->conj[h->cur];
-  int rc = run_kpass(h, a);
+  rc = run_kpass(h, a);
   if (rc) return rc;
   h->inv_valid = false;  // G1..G4 now hold the transforms of `in`
   rc = finish_rhs_into_kout(h, false);
   if (rc) return rc;
-  HIPCHK(hipMemcpy2DAsync(out, h->nyf * sizeof(double2), h->kout, h->ld * sizeof(double2), h->nyf * sizeof(double2),
-                          h->p.nx, hipMemcpyDeviceToHost, h->stream));
+  rc = copy_out(h, out, h->kout, hipMemcpyDeviceToHost);
+  if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
   return NS2D_OK;
 }
@@ -861,7 +890,8 @@ int ns2d_measures(ns2d_t* h, int with_flux, ns2d_measures_t* m, double* enrg_spe
   HIPCHK(hipMemcpyAsync(part.data(), h->meas, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));
   double2 wf = make_double2(0.0, 0.0);
   if (h->ph.force_on)
-    HIPCHK(hipMemcpyAsync(&wf, h->w[h->wcur] + h->p.force_k, sizeof wf, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(&wf, h->w[h->wcur] + spec_idx(0, h->p.force_k, h->ld), sizeof wf, hipMemcpyDeviceToHost,
+                          h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   const double cn = measures_cn(h);
   double tot[NQ] = {0};
@@ -916,8 +946,8 @@ int ns2d_r2c(ns2d_t* h, const double* w, double* w_hat) {
   a.epi = EPI_RHS;
   int rc = run_kpass(h, a);
   if (rc) return rc;
-  HIPCHK(hipMemcpy2DAsync(w_hat, h->nyf * sizeof(double2), h->kout, h->ld * sizeof(double2), h->nyf * sizeof(double2),
-                          h->p.nx, hipMemcpyDeviceToHost, h->stream));
+  rc = copy_out(h, w_hat, h->kout, hipMemcpyDeviceToHost);
+  if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
   return NS2D_OK;
 }
@@ -927,14 +957,14 @@ int ns2d_c2r(ns2d_t* h, const double* w_hat, double* w) {
   HIPCHK(hipSetDevice(h->p.device));
   const size_t n = (size_t)h->p.nx * h->p.ny;
   if (!h->wreal) HIPCHK(hipMalloc((void**)&h->wreal, sizeof(double) * n));
-  HIPCHK(hipMemcpy2DAsync(h->sbuf, h->ld * sizeof(double2), w_hat, h->nyf * sizeof(double2), h->nyf * sizeof(double2),
-                          h->p.nx, hipMemcpyHostToDevice, h->stream));
+  int rc = copy_in(h, h->sbuf, w_hat, hipMemcpyHostToDevice);
+  if (rc) return rc;
   KPassArgs a = kbase(h);
   a.mode = KP_INV;
   a.epi = EPI_FROM_INPUT;
   a.sin = h->sbuf;
   a.conj_dst = nullptr;  // the state's ky=0 hand-off is left alone
-  int rc = run_kpass(h, a);
+  rc = run_kpass(h, a);
   if (rc) return rc;
   h->inv_valid = false;  // G1..G4 now hold the transforms of w_hat
   RPassArgs r = rbase(h, RP_W);
@@ -1208,6 +1238,111 @@ int ens_autotune(ns2d_ensemble* e, int reps) {
   return NS2D_OK;
 }
 
+// ---- tuned schedule cache (JSON Lines, one record per size class) --------
+// {"device": "<arch>/<name>/<CUs>", "tables": "<shape-table hash>", "nx": .., "ny": ..,
+//  "members": .., "kpass": <shape index>, "rpass": <shape index>}
+// File: NS2D_ENS_CACHE (empty: no cache), else $XDG_CACHE_HOME/ns2d/ens_schedule.jsonl
+// or $HOME/.cache/ns2d/ens_schedule.jsonl.  The last matching record wins.
+std::string ens_cache_path() {
+  if (const char* p = std::getenv("NS2D_ENS_CACHE")) return p;
+  std::string base;
+  if (const char* x = std::getenv("XDG_CACHE_HOME"); x && *x) base = x;
+  else if (const char* h = std::getenv("HOME"); h && *h) base = std::string(h) + "/.cache";
+  return base.empty() ? std::string() : base + "/ns2d/ens_schedule.jsonl";
+}
+
+std::string ens_device_key(int dev) {
+  hipDeviceProp_t pr{};
+  if (hipGetDeviceProperties(&pr, dev) != hipSuccess) return "unknown";
+  char buf[512];
+  std::snprintf(buf, sizeof buf, "%s/%s/%d", pr.gcnArchName, pr.name, pr.multiProcessorCount);
+  for (char* c = buf; *c; ++c)
+    if (*c == '"' || *c == '\\') *c = '_';
+  return buf;
+}
+
+// value of "key": in a record line (string without quotes, or a number)
+bool jl_field(const std::string& line, const char* key, std::string* out) {
+  const std::string k = std::string("\"") + key + "\":";
+  size_t p = line.find(k);
+  if (p == std::string::npos) return false;
+  p += k.size();
+  while (p < line.size() && line[p] == ' ') ++p;
+  if (p < line.size() && line[p] == '"') {
+    const size_t q = line.find('"', p + 1);
+    if (q == std::string::npos) return false;
+    *out = line.substr(p + 1, q - p - 1);
+  } else {
+    size_t q = p;
+    while (q < line.size() && (line[q] == '-' || (line[q] >= '0' && line[q] <= '9'))) ++q;
+    if (q == p) return false;
+    *out = line.substr(p, q - p);
+  }
+  return true;
+}
+
+std::string ens_tables_key(const ns2d_ensemble::Class& c) {
+  char buf[32];
+  std::snprintf(buf, sizeof buf, "%016llx", shape_table_hash(c.nx, c.ny));
+  return buf;
+}
+
+// Set the classes' shapes from the cache; true when every class was found.
+bool ens_cache_load(ns2d_ensemble* e) {
+  const std::string path = ens_cache_path();
+  if (path.empty()) return false;
+  FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) return false;
+  const std::string dev = ens_device_key(e->m[0]->p.device);
+  std::vector<int> kv(e->cls.size(), -1), rv(e->cls.size(), -1);
+  char buf[1024];
+  while (std::fgets(buf, sizeof buf, f)) {
+    const std::string line(buf);
+    std::string d, t, nx, ny, nm, k, r;
+    if (!jl_field(line, "device", &d) || !jl_field(line, "tables", &t) || !jl_field(line, "nx", &nx) ||
+        !jl_field(line, "ny", &ny) || !jl_field(line, "members", &nm) || !jl_field(line, "kpass", &k) ||
+        !jl_field(line, "rpass", &r) || d != dev)
+      continue;
+    for (size_t i = 0; i < e->cls.size(); ++i) {
+      const auto& c = e->cls[i];
+      if (std::atoi(nx.c_str()) == c.nx && std::atoi(ny.c_str()) == c.ny &&
+          std::atoi(nm.c_str()) == (int)c.idx.size() && t == ens_tables_key(c)) {
+        const int ki = std::atoi(k.c_str()), ri = std::atoi(r.c_str());
+        if (ki >= 0 && ki < kpass_variant_count(c.nx) && ri >= 0 && ri < rpass_variant_count(c.ny)) {
+          kv[i] = ki;
+          rv[i] = ri;
+        }
+      }
+    }
+  }
+  std::fclose(f);
+  for (size_t i = 0; i < e->cls.size(); ++i)
+    if (kv[i] < 0) return false;
+  for (size_t i = 0; i < e->cls.size(); ++i) {
+    e->cls[i].kvar = kv[i];
+    e->cls[i].rvar = rv[i];
+  }
+  return true;
+}
+
+// Append the classes' tuned shapes to the cache (best effort: an unwritable
+// cache is skipped).
+void ens_cache_store(ns2d_ensemble* e) {
+  const std::string path = ens_cache_path();
+  if (path.empty()) return;
+  for (size_t p = 1; p < path.size(); ++p)
+    if (path[p] == '/') (void)::mkdir(path.substr(0, p).c_str(), 0755);
+  FILE* f = std::fopen(path.c_str(), "a");
+  if (!f) return;
+  const std::string dev = ens_device_key(e->m[0]->p.device);
+  for (const auto& c : e->cls)
+    std::fprintf(f,
+                 "{\"device\": \"%s\", \"tables\": \"%s\", \"nx\": %d, \"ny\": %d, \"members\": %d, "
+                 "\"kpass\": %d, \"rpass\": %d}\n",
+                 dev.c_str(), ens_tables_key(c).c_str(), c.nx, c.ny, (int)c.idx.size(), c.kvar, c.rvar);
+  std::fclose(f);
+}
+
 void ens_advance(ns2d_ensemble* e) {
   for (ns2d_handle* h : e->m) {
     after_rk_update(h);
@@ -1306,10 +1441,15 @@ int ns2d_ens_step(ns2d_ens_t* e, const double* dt, long nsteps) {
     ++s;
   }
   if (s < nsteps && !e->tuned) {
+    // NS2D_ENS_TUNE: 0 keeps the plain solver's shapes, 2 re-tunes ignoring
+    // the schedule cache, otherwise a cached schedule for these classes on
+    // this device is used and a missing one is tuned and cached
     const char* t = std::getenv("NS2D_ENS_TUNE");
-    if (!t || std::atoi(t) != 0) {
+    const int mode = t ? std::atoi(t) : 1;
+    if (mode != 0 && (mode == 2 || !ens_cache_load(e))) {
       int rc = ens_autotune(e, 3);
       if (rc) return rc;
+      ens_cache_store(e);
     }
     e->tuned = true;
   }
@@ -1380,7 +1520,9 @@ int ns2d_ens_autotune(ns2d_ens_t* e, int reps) {
     int rc = prepare_inv(h);
     if (rc) return rc;
   }
-  return ens_autotune(e, reps);
+  int rc = ens_autotune(e, reps);
+  if (rc == NS2D_OK) ens_cache_store(e);
+  return rc;
 }
 
 int ns2d_ens_variants(ns2d_ens_t* e, int member, int* kpass_idx, int* rpass_idx) {
@@ -1715,13 +1857,14 @@ int ns2d_vjp_device(ns2d_t* h, const void* d_w0, double dt, long nsteps, const v
   HIPCHK(hipSetDevice(h->p.device));
   int rc = adj_alloc(h, 1, 1);
   if (rc) return rc;
-  const size_t rowb = h->nyf * sizeof(double2), pitch = h->ld * sizeof(double2);
-  HIPCHK(hipMemcpy2DAsync(h->w[h->wcur], pitch, d_w0, rowb, rowb, h->p.nx, hipMemcpyDeviceToDevice, h->stream));
-  HIPCHK(hipMemcpy2DAsync(h->lamp[0], pitch, d_lam_T, rowb, rowb, h->p.nx, hipMemcpyDeviceToDevice, h->stream));
+  rc = copy_in(h, h->w[h->wcur], d_w0, hipMemcpyDeviceToDevice);
+  if (rc == NS2D_OK) rc = copy_in(h, h->lamp[0], d_lam_T, hipMemcpyDeviceToDevice);
+  if (rc) return rc;
   double2* res = nullptr;
   rc = vjp_impl(h, dt, nsteps, ckpt_every, &res);
   if (rc) return rc;
-  HIPCHK(hipMemcpy2DAsync(d_grad_w0, rowb, res, pitch, rowb, h->p.nx, hipMemcpyDeviceToDevice, h->stream));
+  rc = copy_out(h, d_grad_w0, res, hipMemcpyDeviceToDevice);
+  if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
   return NS2D_OK;
 }
@@ -1733,13 +1876,14 @@ int ns2d_vjp(ns2d_t* h, const double* w0, double dt, long nsteps, const double* 
   HIPCHK(hipSetDevice(h->p.device));
   int rc = adj_alloc(h, 1, 1);
   if (rc) return rc;
-  const size_t rowb = h->nyf * sizeof(double2), pitch = h->ld * sizeof(double2);
-  HIPCHK(hipMemcpy2DAsync(h->w[h->wcur], pitch, w0, rowb, rowb, h->p.nx, hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipMemcpy2DAsync(h->lamp[0], pitch, lam_T, rowb, rowb, h->p.nx, hipMemcpyHostToDevice, h->stream));
+  rc = copy_in(h, h->w[h->wcur], w0, hipMemcpyHostToDevice);
+  if (rc == NS2D_OK) rc = copy_in(h, h->lamp[0], lam_T, hipMemcpyHostToDevice);
+  if (rc) return rc;
   double2* res = nullptr;
   rc = vjp_impl(h, dt, nsteps, ckpt_every, &res);
   if (rc) return rc;
-  HIPCHK(hipMemcpy2DAsync(grad_w0, rowb, res, pitch, rowb, h->p.nx, hipMemcpyDeviceToHost, h->stream));
+  rc = copy_out(h, grad_w0, res, hipMemcpyDeviceToHost);
+  if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
   return NS2D_OK;
 }

@@ -107,8 +107,66 @@ __device__ __forceinline__ void apply_twiddles(double2* v, double2 w) {
   }
 }
 
-template <int N, int TL, int SIGN, bool BLOCK, int P, int PS>
-__device__ __forceinline__ void fft_reg_pass(double2* v, double2* buf, const double2* __restrict__ tw, int l) {
+// Twiddles of one thread's butterflies, loaded once per kernel: the passes
+// with span P > 1 use E/R of them each (one per butterfly), the same in both
+// directions (conjugated for the inverse).  Preloading them keeps the table
+// loads out of the transforms' barrier-separated passes.
+template <int N, int TL, int P>
+constexpr int tw_count() {
+  if constexpr (N / P <= 1) {
+    return 0;
+  } else {
+    constexpr int E = N / TL, REM = N / P;
+    constexpr int R0 = E < 16 ? E : 16;
+    constexpr int R = R0 < REM ? R0 : REM;
+    return (P > 1 ? E / R : 0) + tw_count<N, TL, P * R>();
+  }
+}
+
+template <int N, int TL>
+struct TwReg {
+  static constexpr int n = tw_count<N, TL, 1>();
+  double2 w[n > 0 ? n : 1];
+};
+
+template <int N, int TL, int P, int OFF>
+__device__ __forceinline__ void load_twiddles_pass(TwReg<N, TL>& t, const double2* __restrict__ tw, int l) {
+  constexpr int E = N / TL, REM = N / P;
+  if constexpr (REM > 1) {
+    constexpr int R0 = E < 16 ? E : 16;
+    constexpr int R = R0 < REM ? R0 : REM;
+    constexpr int NB = E / R;
+    constexpr int TWS = NS2D_TW_LEN / (P * R);
+    if constexpr (P > 1) {
+#pragma unroll
+      for (int m = 0; m < NB; ++m) t.w[OFF + m] = tw[((l + TL * m) & (P - 1)) * TWS];
+    }
+    load_twiddles_pass<N, TL, P * R, OFF + (P > 1 ? NB : 0)>(t, tw, l);
+  }
+}
+
+// t.w = this thread's twiddles (lane l of a line owned by TL threads)
+template <int N, int TL>
+__device__ __forceinline__ void load_twiddles(TwReg<N, TL>& t, const double2* __restrict__ tw, int l) {
+  load_twiddles_pass<N, TL, 1, 0>(t, tw, l);
+}
+
+// twiddle source: the global table, or the thread's preloaded set
+template <int SIGN, int N, int TL>
+__device__ __forceinline__ double2 tw_get(const double2* __restrict__ tw, int idx, int) {
+  return twiddle<SIGN>(tw, idx);
+}
+template <int SIGN, int N, int TL>
+__device__ __forceinline__ double2 tw_get(const TwReg<N, TL>& t, int, int slot) {
+  double2 w = t.w[slot];
+  // opaque at the point of use: the powers w^r of the pass are formed here,
+  // not hoisted to the kernel start where they would all stay live
+  asm volatile("" : "+v"(w.x), "+v"(w.y));
+  return SIGN < 0 ? w : make_double2(w.x, -w.y);
+}
+
+template <int N, int TL, int SIGN, bool BLOCK, int P, int PS, int OFF, typename TW>
+__device__ __forceinline__ void fft_reg_pass(double2* v, double2* buf, const TW& tw, int l) {
   constexpr int E = N / TL;
   constexpr int REM = N / P;
   if constexpr (REM > 1) {
@@ -124,7 +182,7 @@ __device__ __forceinline__ void fft_reg_pass(double2* v, double2* buf, const dou
 #pragma unroll
       for (int r = 0; r < R; ++r) x[r] = v[m + r * NB];
       const int k = i & (P - 1);
-      if constexpr (P > 1) apply_twiddles<R>(x, twiddle<SIGN>(tw, k * TWS));
+      if constexpr (P > 1) apply_twiddles<R>(x, tw_get<SIGN, N, TL>(tw, k * TWS, OFF + m));
       dftr<R, SIGN>(x);
       if constexpr (LAST) {
 #pragma unroll
@@ -140,7 +198,7 @@ __device__ __forceinline__ void fft_reg_pass(double2* v, double2* buf, const dou
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = buf[padx_s<PS>(l + TL * e)];
       xsync<BLOCK>();
-      fft_reg_pass<N, TL, SIGN, BLOCK, P * R, PS>(v, buf, tw, l);
+      fft_reg_pass<N, TL, SIGN, BLOCK, P * R, PS, OFF + (P > 1 ? NB : 0)>(v, buf, tw, l);
     }
   }
 }
@@ -149,11 +207,13 @@ __device__ __forceinline__ void fft_reg_pass(double2* v, double2* buf, const dou
 // on exit.  buf: this line's LDS exchange buffer (padx_s<PS>(N) elements).
 // All TL threads of the line (and, with BLOCK, every thread of the block)
 // must call.
-template <int N, int TL, int SIGN, bool BLOCK, int PS = 3>
-__device__ __forceinline__ void fft_reg(double2* v, double2* buf, const double2* __restrict__ tw, int l) {
+// tw: the global table (const double2*) or the thread's preloaded TwReg<N, TL>
+// (loaded with the same l).
+template <int N, int TL, int SIGN, bool BLOCK, int PS = 3, typename TW>
+__device__ __forceinline__ void fft_reg(double2* v, double2* buf, const TW& tw, int l) {
   static_assert((N & (N - 1)) == 0 && N >= 2 && N % TL == 0, "bad FFT shape");
   static_assert(N <= NS2D_TW_LEN, "twiddle table too short");
-  fft_reg_pass<N, TL, SIGN, BLOCK, 1, PS>(v, buf, tw, l);
+  fft_reg_pass<N, TL, SIGN, BLOCK, 1, PS, 0>(v, buf, tw, l);
 }
 
 }  // namespace ns2d

@@ -3,7 +3,23 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+// Spectral arrays ([Nx][ld] complex, kx rows, ky columns) are stored in
+// x-tiles of NS2D_XTILE rows: element (x, ky) at ((x/XT)*ld + ky)*XT + x%XT,
+// so XT consecutive rows of a column are contiguous (XT = 1: plain row-major).
+#ifndef NS2D_XTILE
+#define NS2D_XTILE 1
+#endif
+
 namespace ns2d {
+
+constexpr int XT = NS2D_XTILE;
+constexpr int XT_SHIFT = XT == 1 ? 0 : XT == 2 ? 1 : XT == 4 ? 2 : XT == 8 ? 3 : -1;
+static_assert(XT_SHIFT >= 0, "NS2D_XTILE must be 1, 2, 4 or 8");
+
+__host__ __device__ __forceinline__ size_t spec_idx(int x, int ky, int ld) {
+  if constexpr (XT == 1) return (size_t)x * ld + ky;
+  else return (((size_t)(x >> XT_SHIFT) * ld + ky) << XT_SHIFT) + (size_t)(x & (XT - 1));
+}
 
 // RK4 Butcher tableau and AB4 weights (reference Solver/src/solver.c:29-47)
 constexpr double RK4_A21 = 0.5, RK4_A32 = 0.5, RK4_A43 = 1.0;
@@ -142,9 +158,12 @@ int rpass_variant_count(int ny);
 int rpass_variant_info(int ny, int idx, int* TLR, int* E, int* T, int* PAIRS);
 int rpass_variant_extra(int ny, int idx, int* W, int* PS, int* OFF, int* PK);
 int kpass_variant_extra(int nx, int idx, int* W, int* PS, int* OFF, int* PK);
+unsigned long long shape_table_hash(int nx, int ny);
 hipError_t launch_r2c_rows(int nx, int ny, const double* in, double2* out, const double2* tw, int ld,
                            hipStream_t s);
 hipError_t launch_measures(int nbins, const MeasArgs& a, hipStream_t s);
+hipError_t launch_relayout(const double2* src, double2* dst, int nx, int nyf, int ld, int dir, hipStream_t s);
+
 hipError_t launch_kpass_adj(int nx, const KAdjArgs& a, hipStream_t s);
 hipError_t launch_rpass_adj(int nx, int ny, const RAdjArgs& a, hipStream_t s);
 

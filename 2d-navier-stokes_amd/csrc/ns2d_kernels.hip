@@ -28,6 +28,31 @@ const std::vector<KVariant>& kvariant_table() {
   return t;
 }
 
+// FNV-1a hash of the compiled column-pass shapes of length nx and row-pass
+// shapes of length ny (a tuned schedule names shapes by index, valid only for
+// the same tables)
+unsigned long long shape_table_hash(int nx, int ny) {
+  unsigned long long h = 1469598103934665603ull;
+  auto mix = [&](int x) {
+    for (int b = 0; b < 4; ++b) {
+      h ^= (unsigned long long)((x >> (8 * b)) & 0xff);
+      h *= 1099511628211ull;
+    }
+  };
+  for (const KVariant& v : kvariant_table())
+    if (v.L == nx) {
+      const int f[] = {v.L, v.C, v.T, v.KS, v.PS, v.OFF, v.PK, v.W, v.PF};
+      for (int x : f) mix(x);
+    }
+  mix(-1);
+  for (const RVariant& v : rvariant_table())
+    if (v.NY == ny) {
+      const int f[] = {v.NY, v.TLR, v.PAIRS, v.W, v.PFR};
+      for (int x : f) mix(x);
+    }
+  return h;
+}
+
 const std::vector<RVariant>& rvariant_table() {
   static const std::vector<RVariant> t = [] {
     std::vector<RVariant> v;
@@ -141,7 +166,7 @@ static const RVariant* find_rvariant(int ny, int idx) {
 hipError_t launch_rpass_batched(int nx, int ny, int variant, const RPassArgs* tab, int members, hipStream_t s) {
   const RVariant* v = find_rvariant(ny, variant);
   if (!v) return hipErrorInvalidValue;
-  const int units = v->PFR == 2 ? nx : nx / 2;   // rows or row pairs
+  const int units = v->PFR >= 2 ? nx : nx / 2;   // rows or row pairs
   const int per = (units + v->PAIRS - 1) / v->PAIRS;
   hipLaunchKernelGGL(v->batched, dim3(per * members), dim3(v->TLR * v->PAIRS), 0, s, tab, per);
   return hipGetLastError();
@@ -169,14 +194,14 @@ int rpass_variant_extra(int ny, int idx, int* W, int* PS, int* OFF, int* PK) {
   *W = v->W;
   *PS = 3;
   *OFF = 1;
-  *PK = ((v->PFR == 1) << 1) | ((v->PFR == 2) << 3);
+  *PK = ((v->PFR == 1 || v->PFR == 3) << 1) | ((v->PFR >= 2) << 3);
   return 0;
 }
 
 hipError_t launch_rpass(int nx, int ny, int variant, const RPassArgs& a, hipStream_t s) {
   const RVariant* v = find_rvariant(ny, variant);
   if (!v) return hipErrorInvalidValue;
-  const int units = v->PFR == 2 ? nx : nx / 2;   // rows or row pairs
+  const int units = v->PFR >= 2 ? nx : nx / 2;   // rows or row pairs
   hipLaunchKernelGGL(v->kern, dim3((units + v->PAIRS - 1) / v->PAIRS), dim3(v->TLR * v->PAIRS), 0, s, a);
   return hipGetLastError();
 }
@@ -209,6 +234,24 @@ hipError_t launch_r2c_rows(int nx, int ny, const double* in, double2* out, const
     case 4096: return launch_r2c_rows_t<4096>(nx, in, out, tw, ld, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+// Between the host layout [Nx][nyf] (contiguous rows, what callers pass) and
+// the device layout (spec_idx, padding columns ky >= nyf zero): dir 0 packs
+// src (host layout) into dst, dir 1 unpacks src into dst (host layout).
+__global__ __launch_bounds__(256) void relayout_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                       int nyf, int ld, int dir) {
+  const int x = blockIdx.y, ky = blockIdx.x * 256 + threadIdx.x;
+  if (dir == 0) {
+    if (ky < ld) dst[spec_idx(x, ky, ld)] = ky < nyf ? src[(size_t)x * nyf + ky] : make_double2(0.0, 0.0);
+  } else if (ky < nyf) {
+    dst[(size_t)x * nyf + ky] = src[spec_idx(x, ky, ld)];
+  }
+}
+
+hipError_t launch_relayout(const double2* src, double2* dst, int nx, int nyf, int ld, int dir, hipStream_t s) {
+  hipLaunchKernelGGL(relayout_kernel, dim3((ld + 255) / 256, nx), dim3(256), 0, s, src, dst, nyf, ld, dir);
+  return hipGetLastError();
 }
 
 hipError_t launch_measures(int nbins, const MeasArgs& a, hipStream_t s) {

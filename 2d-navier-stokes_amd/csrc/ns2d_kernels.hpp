@@ -37,7 +37,22 @@
 #include <cstdlib>
 #include <vector>
 
+// NS2D_TW_PRELOAD = 1: each thread loads its FFT twiddles once per kernel
+// (TwReg) instead of from the table inside every transform pass.
+#ifndef NS2D_TW_PRELOAD
+#define NS2D_TW_PRELOAD 1
+#endif
+
 namespace ns2d {
+
+#if NS2D_TW_PRELOAD
+#define NS2D_TWIDDLES(N, TLN, name, table, lane) \
+  TwReg<N, TLN> name##_reg_;                     \
+  load_twiddles<N, TLN>(name##_reg_, table, lane); \
+  const TwReg<N, TLN>& name = name##_reg_;
+#else
+#define NS2D_TWIDDLES(N, TLN, name, table, lane) const double2* const name = table;
+#endif
 
 __device__ __forceinline__ int kx_of_row(int i, int nx) { return i <= nx / 2 ? i : i - nx; }
 
@@ -69,7 +84,7 @@ __device__ __forceinline__ double dealias_factor(const PhysArgs& ph, int kx, int
       return q == 0 ? 1.0 : (q == 1 ? ph.houli_edge : ph.houli_corner);
     }
     case FILTER_HOULI:   // exp(-36 (|k|/k_N)^36), tabulated on the host
-      return ph.filt_tab[(kx < 0 ? kx + ph.nx : kx) * ph.ld + ky];
+      return ph.filt_tab[spec_idx(kx < 0 ? kx + ph.nx : kx, ky, ph.ld)];
     default: return 1.0;
   }
 }
@@ -135,9 +150,9 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
   const bool pair = PK && ky == 0;
   const bool pblk = PK && g == 0;  // uniform per workgroup
   const bool writer = split == 0;
-  const double2* __restrict__ tw = a.tw;
   const PhysArgs& ph = a.ph;
   double2* buf = lds + c * LS;
+  NS2D_TWIDDLES(L, TL, tw, a.tw, l)
   // the pair's held columns (see above)
   double2* __restrict__ hold = a.pairbuf + (size_t)split * 2 * L;
 
@@ -169,7 +184,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
       for (int it = 0; it < E; ++it) {
         const int i = l + TL * it;
         const int kx = kx_of_row(i, nx);
-        const size_t idx = (size_t)i * ld + ky;
+        const size_t idx = spec_idx(i, ky, ld);
         wpre[it] = !pre ? make_double2(0.0, 0.0) : (cf && kx < 0) ? cconj(a.conj_src[-kx]) : a.w_in[idx];
         apre[it] = (pre && need_acc) ? a.acc[idx] : make_double2(0.0, 0.0);
         lpre[it] = (pre && need_lam) ? a.lam[idx] : 0.0;
@@ -182,8 +197,8 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int x = l + TL * e;
-      if (pair) R[e] = make_double2(a.nin[(size_t)x * ld].x, a.nin[(size_t)x * ld + H].x);
-      else R[e] = colv ? a.nin[(size_t)x * ld + ky] : make_double2(0.0, 0.0);
+      if (pair) R[e] = make_double2(a.nin[spec_idx(x, 0, ld)].x, a.nin[spec_idx(x, H, ld)].x);
+      else R[e] = colv ? a.nin[spec_idx(x, ky, ld)] : make_double2(0.0, 0.0);
       if ((e & 3) == 3) asm volatile("" ::: "memory");
     }
     fft_reg<L, TL, -1, BLOCK, PS>(R, buf, tw, opaque(l));
@@ -218,7 +233,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
     // held live across it
     const int kq = opaque(q ? H : ky);
     const int lq = opaque(l);
-    auto index_of = [&](int it) -> size_t { return (size_t)(lq + TL * it) * ld + kq; };
+    auto index_of = [&](int it) -> size_t { return spec_idx(lq + TL * it, kq, ld); };
     if (PKR && q == 1) {
       // S_0 to RH, KH (forward modes) to R
 #pragma unroll
@@ -240,7 +255,10 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
         double2 v = R[e];
         if (a.mode & KP_RAW) {
           // plain unnormalised forward transform (initial conditions)
-        } else if (kx == 0 && kq == 0) {
+        } else if ((kx == 0 && kq == 0) || !colv) {
+          // (0,0) = 0; a slot past the last column holds no mode, and its ky
+          // may lie past the row's end (ky >= ld), where the per-mode filter
+          // table must not be read
           v = make_double2(0.0, 0.0);
         } else {
           v = cscale(nrm, v);                                  // solver.c:1005
@@ -257,7 +275,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
       const int i = lq + TL * it;
       const int kx = kx_of_row(i, nx);
       if (conj_fix && kx < 0) return cconj(a.conj_src[-kx]);
-      return a.w_in[(size_t)i * ld + kq];
+      return a.w_in[spec_idx(i, kq, ld)];
     };
     auto acc_at = [&](int it) -> double2 {
       if constexpr (PF != 0) return apre[it];
@@ -450,22 +468,22 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
       double* __restrict__ g4 = reinterpret_cast<double*>(a.g[3]);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const size_t r = (size_t)(l_o + TL * e) * ld;
+        const size_t r0 = spec_idx(l_o + TL * e, 0, ld), rh = spec_idx(l_o + TL * e, H, ld);
         if (f == 0) {
-          gout[r] = make_double2(0.0, 0.0);
-          gout[r + H] = make_double2(0.0, F[e].x);
-          g4[2 * (r + H) + 1] = F[e].y;
+          gout[r0] = make_double2(0.0, 0.0);
+          gout[rh] = make_double2(0.0, F[e].x);
+          g4[2 * rh + 1] = F[e].y;
         } else if (f == 3) {
-          gout[r] = make_double2(F[e].x, 0.0);
-          g4[2 * (r + H)] = F[e].y;
+          gout[r0] = make_double2(F[e].x, 0.0);
+          g4[2 * rh] = F[e].y;
         } else {
-          gout[r] = make_double2(F[e].x, 0.0);
-          gout[r + H] = make_double2(F[e].y, 0.0);
+          gout[r0] = make_double2(F[e].x, 0.0);
+          gout[rh] = make_double2(F[e].y, 0.0);
         }
       }
     } else if (colv) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) gout[(size_t)(l_o + TL * e) * ld + ky_o] = F[e];
+      for (int e = 0; e < E; ++e) gout[spec_idx(l_o + TL * e, ky_o, ld)] = F[e];
     }
   }
 }
@@ -505,7 +523,8 @@ __global__ __launch_bounds__(T, W) void kpass_batched(const KPassArgs* __restric
 // (imaginary part zero) and N(x0,k) = W[k] with the imaginary parts of k = 0
 // and NY/2 set to zero, as a real-input transform gives them.  Twice the
 // workgroups and three transforms per workgroup instead of five, for one
-// more forward transform per row pair.
+// more forward transform per row pair.  PFR = 3: a single row whose four G
+// fields are all loaded before its first transform.
 template <int NY, int TLR, int PAIRS, int PFR>
 __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   constexpr int T = TLR * PAIRS;
@@ -513,7 +532,7 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   constexpr int E = NY / TLR;
   constexpr int LS = line_stride(NY);
   constexpr bool BLOCK = TLR > 64;
-  constexpr bool SR = PFR == 2;
+  constexpr bool SR = PFR >= 2;
   static_assert(E >= 2 && (BLOCK ? PAIRS == 1 : 64 % TLR == 0), "bad rpass shape");
   static_assert(!SR || PAIRS == 1, "single-row shapes own one row per workgroup");
   __shared__ double2 lds[PAIRS * LS];
@@ -524,8 +543,8 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   const int x0 = SR ? blk : 2 * (blk * PAIRS + q);
   const bool valid = x0 < a.nx;   // uniform per block when BLOCK
   const int ld = a.ld;
-  const double2* __restrict__ tw = a.tw;
   double2* buf = lds + q * LS;
+  NS2D_TWIDDLES(NY, TLR, tw, a.tw, l)
 
   // full-length spectrum of A + iB for row x (which: 0 -> u + iv, 1 -> w_x + i w_y,
   // 2 -> w(x) + i w(x+1), 3 -> w(x))
@@ -538,18 +557,18 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
       double2 A, B;
       const double fk = (double)kk;
       if (which == 0) {         // u = i ky G1, v = G2
-        const double2 g1 = a.g[0][(size_t)x * ld + kk];
+        const double2 g1 = a.g[0][spec_idx(x, kk, ld)];
         A = make_double2(-fk * g1.y, fk * g1.x);
-        B = a.g[1][(size_t)x * ld + kk];
+        B = a.g[1][spec_idx(x, kk, ld)];
       } else if (which == 1) {  // w_x = G3, w_y = i ky G4
-        A = a.g[2][(size_t)x * ld + kk];
-        const double2 g4 = a.g[3][(size_t)x * ld + kk];
+        A = a.g[2][spec_idx(x, kk, ld)];
+        const double2 g4 = a.g[3][spec_idx(x, kk, ld)];
         B = make_double2(-fk * g4.y, fk * g4.x);
       } else if (which == 2) {  // w of rows x and x+1 = G4
-        A = a.g[3][(size_t)x * ld + kk];
-        B = a.g[3][(size_t)(x + 1) * ld + kk];
+        A = a.g[3][spec_idx(x, kk, ld)];
+        B = a.g[3][spec_idx(x + 1, kk, ld)];
       } else {                  // w of row x alone
-        A = a.g[3][(size_t)x * ld + kk];
+        A = a.g[3][spec_idx(x, kk, ld)];
         B = make_double2(0.0, 0.0);
       }
       if (kk == 0 || kk == H) { A.y = 0.0; B.y = 0.0; }
@@ -571,7 +590,7 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
           for (int e = 0; e < E; ++e) {
             const int ky = l + TLR * e;
             const int kk = ky > H ? NY - ky : ky;
-            pre[r][f][e] = a.g[f][(size_t)(x0 + r) * ld + kk];
+            pre[r][f][e] = a.g[f][spec_idx(x0 + r, kk, ld)];
           }
       // the same assembly as build() from the loaded values
       auto build_pre = [&](double2* z, int which, const double2 (&g)[4][E]) {
@@ -611,8 +630,9 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   } else if (SR && (a.mode & RP_NL) && (BLOCK || valid)) {
     double2 z1[E], z2[E];
     build(z1, 0, x0, opaque(l));
+    if constexpr (PFR == 3) build(z2, 1, x0, opaque(l));   // both field pairs loaded up front
     fft_reg<NY, TLR, +1, BLOCK>(z1, buf, tw, opaque(l));
-    build(z2, 1, x0, opaque(l));
+    if constexpr (PFR != 3) build(z2, 1, x0, opaque(l));
     fft_reg<NY, TLR, +1, BLOCK>(z2, buf, tw, opaque(l));
 #pragma unroll
     for (int e = 0; e < E; ++e) w[e] = make_double2(z1[e].x * z2[e].x + z1[e].y * z2[e].y, 0.0);
@@ -674,11 +694,10 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
 
   if (SR && (a.mode & RP_NL) && (BLOCK || valid)) {
     fft_reg<NY, TLR, -1, BLOCK>(w, buf, tw, opaque(l));
-    double2* __restrict__ out0 = a.nout + (size_t)x0 * ld;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = l + TLR * e;
-      if (k <= H) out0[k] = (k == 0 || k == H) ? make_double2(w[e].x, 0.0) : w[e];
+      if (k <= H) a.nout[spec_idx(x0, k, ld)] = (k == 0 || k == H) ? make_double2(w[e].x, 0.0) : w[e];
     }
   } else if ((a.mode & RP_NL) && (BLOCK || valid)) {
     fft_reg<NY, TLR, -1, BLOCK>(w, buf, tw, opaque(l));
@@ -686,16 +705,14 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
 #pragma unroll
     for (int e = 0; e < E; ++e) buf[padx(l + TLR * e)] = w[e];
     xsync<BLOCK>();
-    double2* __restrict__ out0 = a.nout + (size_t)x0 * ld;
-    double2* __restrict__ out1 = out0 + ld;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = l + TLR * e;
       if (k <= H) {
         const double2 p = w[e];
         const double2 qm = cconj(buf[padx((NY - k) & (NY - 1))]);
-        out0[k] = make_double2(0.5 * (p.x + qm.x), 0.5 * (p.y + qm.y));
-        out1[k] = make_double2(0.5 * (p.y - qm.y), -0.5 * (p.x - qm.x));
+        a.nout[spec_idx(x0, k, ld)] = make_double2(0.5 * (p.x + qm.x), 0.5 * (p.y + qm.y));
+        a.nout[spec_idx(x0 + 1, k, ld)] = make_double2(0.5 * (p.y - qm.y), -0.5 * (p.x - qm.x));
       }
     }
   }
@@ -748,19 +765,21 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
   const bool valid = x0 < a.nx;
   if (!BLOCK && !valid) return;
   const int ld = a.ld;
-  const double2* __restrict__ tw = a.tw;
   double2* buf = lds + q * LS;
+  NS2D_TWIDDLES(NY, TLR, tw, a.tw, l)
 
-  // full-length spectrum of A + iB from the half spectra of two real fields
-  // (fb == nullptr only where hasb is 0: B = 0)
-  auto build = [&](double2* z, const double2* fa, const double2* fb, int mula, int mulb, int lo, int hasb = 1) {
+  // full-length spectrum of A + iB from the half spectra of two real fields,
+  // A from row xa of fa, B from row xb of fb (fb == nullptr only where hasb
+  // is 0: B = 0)
+  auto build = [&](double2* z, const double2* fa, int xa, const double2* fb, int xb, int mula, int mulb, int lo,
+                   int hasb = 1) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int ky = lo + TLR * e;
       const bool mirror = ky > H;
       const int kk = mirror ? NY - ky : ky;
       const double fk = (double)kk;
-      double2 A = fa[kk], B = hasb ? fb[kk] : make_double2(0.0, 0.0);
+      double2 A = fa[spec_idx(xa, kk, ld)], B = hasb ? fb[spec_idx(xb, kk, ld)] : make_double2(0.0, 0.0);
       if (mula) A = make_double2(-fk * A.y, fk * A.x);   // times i ky
       if (mulb) B = make_double2(-fk * B.y, fk * B.x);
       if (kk == 0 || kk == H) { A.y = 0.0; B.y = 0.0; }
@@ -771,7 +790,7 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
   // r2c of the real and imaginary parts of a transformed packed row:
   // P[k] = (Z[k] + conj Z[-k])/2, Q[k] = (Z[k] - conj Z[-k])/(2i); writes
   // W2 (mul ? -i k : 1) P to pout and W2 (mulq ? -i k : 1) Q to qout
-  auto split_store = [&](double2* z, double2* pout, double2* qout, int mulp, int mulq, int lo) {
+  auto split_store = [&](double2* z, double2* pout, double2* qout, int x, int mulp, int mulq, int lo) {
 #pragma unroll
     for (int e = 0; e < E; ++e) buf[padx(lo + TLR * e)] = z[e];
     xsync<BLOCK>();
@@ -787,8 +806,8 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
         const double fk = (double)k;
         if (mulp) P = make_double2(fk * P.y, -fk * P.x);   // times -i k
         if (mulq) Q = make_double2(fk * Q.y, -fk * Q.x);
-        pout[k] = cscale(w2, P);
-        qout[k] = cscale(w2, Q);
+        pout[spec_idx(x, k, ld)] = cscale(w2, P);
+        qout[spec_idx(x, k, ld)] = cscale(w2, Q);
       }
     }
     xsync<BLOCK>();
@@ -796,17 +815,16 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
 
   // l_n of rows x0 and x0+1 (SR: of row x0) from the x-transformed r2c^T field L
   double2 zl[E];
-  if constexpr (SR) build(zl, a.lin + (size_t)x0 * ld, nullptr, 0, 0, opaque(l), 0);
-  else build(zl, a.lin + (size_t)x0 * ld, a.lin + (size_t)(x0 + 1) * ld, 0, 0, opaque(l));
+  if constexpr (SR) build(zl, a.lin, x0, nullptr, x0, 0, 0, opaque(l), 0);
+  else build(zl, a.lin, x0, a.lin, x0 + 1, 0, 0, opaque(l));
   fft_reg<NY, TLR, +1, BLOCK>(zl, buf, tw, opaque(l));
 #pragma unroll 1
   for (int r = 0; r < (SR ? 1 : 2); ++r) {
     const int x = opaque(x0 + r);
-    const size_t row = (size_t)x * ld;
     double2 z1[E], z2[E];
-    build(z1, a.g[0] + row, a.g[1] + row, 1, 0, opaque(l));   // u + i v
+    build(z1, a.g[0], x, a.g[1], x, 1, 0, opaque(l));   // u + i v
     fft_reg<NY, TLR, +1, BLOCK>(z1, buf, tw, opaque(l));
-    build(z2, a.g[2] + row, a.g[3] + row, 0, 1, opaque(l));   // w_x + i w_y
+    build(z2, a.g[2], x, a.g[3], x, 0, 1, opaque(l));   // w_x + i w_y
     fft_reg<NY, TLR, +1, BLOCK>(z2, buf, tw, opaque(l));
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -816,9 +834,9 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
       z2[e] = make_double2(ln * wv.y, ln * u.y);   // l_n w_y + i l_n v
     }
     fft_reg<NY, TLR, -1, BLOCK>(z1, buf, tw, opaque(l));
-    split_store(z1, a.bout[0] + row, a.bout[1] + row, 1, 0, opaque(l));
+    split_store(z1, a.bout[0], a.bout[1], x, 1, 0, opaque(l));
     fft_reg<NY, TLR, -1, BLOCK>(z2, buf, tw, opaque(l));
-    split_store(z2, a.bout[2] + row, a.bout[3] + row, 0, 1, opaque(l));
+    split_store(z2, a.bout[2], a.bout[3], x, 0, 1, opaque(l));
   }
 }
 
@@ -843,10 +861,10 @@ __global__ __launch_bounds__(T) void kpass_adj_kernel(KAdjArgs a) {
   const int ky = ky0 + c;
   const int nyf = a.nyf, ld = a.ld, nx = L;
   const bool colv = ky < nyf;
-  const double2* __restrict__ tw = a.tw;
   const PhysArgs& ph = a.ph;
   double2* buf = lds + c * LS;
   const double h = a.dt;
+  NS2D_TWIDDLES(L, TL, tw, a.tw, l)
 
   double2 LSacc[E];   // N'(S)^T l_K of the stage just finished
 #pragma unroll
@@ -858,7 +876,7 @@ __global__ __launch_bounds__(T) void kpass_adj_kernel(KAdjArgs a) {
       double2 F[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        F[e] = colv ? a.bin[f][(size_t)(l_o + TL * e) * ld + ky_o] : make_double2(0.0, 0.0);
+        F[e] = colv ? a.bin[f][spec_idx(l_o + TL * e, ky_o, ld)] : make_double2(0.0, 0.0);
         if ((e & 3) == 3) asm volatile("" ::: "memory");
       }
       fft_reg<L, TL, -1, BLOCK>(F, buf, tw, l_o);
@@ -885,7 +903,7 @@ __global__ __launch_bounds__(T) void kpass_adj_kernel(KAdjArgs a) {
     LK[e] = make_double2(0.0, 0.0);
     if (!colv) continue;
     const int kx = kx_of_row(l + TL * e, nx);
-    const size_t idx = (size_t)(l + TL * e) * ld + ky;
+    const size_t idx = spec_idx(l + TL * e, ky, ld);
     const bool zero = (kx == 0 && ky == 0);
     const double D = h * a.lam[idx];
     const double A = (2.0 - D) / (2.0 + D), B = 2.0 * h / (2.0 + D);
@@ -940,7 +958,7 @@ __global__ __launch_bounds__(T) void kpass_adj_kernel(KAdjArgs a) {
     fft_reg<L, TL, +1, BLOCK>(LK, buf, tw, l_o);
     if (colv) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) a.lout[(size_t)(l_o + TL * e) * ld + ky_o] = LK[e];
+      for (int e = 0; e < E; ++e) a.lout[spec_idx(l_o + TL * e, ky_o, ld)] = LK[e];
     }
   }
 }
@@ -980,7 +998,7 @@ __global__ __launch_bounds__(T) void r2c_rows_kernel(const double* __restrict__ 
         const double2 O = make_double2(0.5 * (p.y - q.y), -0.5 * (p.x - q.x));
         X = cadd(E, cmul(tw[k * TWS], O));
       }
-      out[(size_t)x * ld + k] = X;
+      out[spec_idx(x, k, ld)] = X;
     }
   }
 }
@@ -1018,7 +1036,7 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
       if (yhi > H) yhi = H;
       const int i = kx >= 0 ? kx : kx + nx;
       for (long ky = ylo; ky <= yhi; ++ky) {
-        const size_t idx = (size_t)i * a.ld + ky;
+        const size_t idx = spec_idx(i, (int)ky, a.ld);
         const double2 w = a.w_hat[idx];
         const double k2 = (double)(kx2 + ky * ky);
         const double c = (ky == 0 || ky == H) ? 1.0 : 2.0;
@@ -1190,7 +1208,8 @@ struct KVariant {
 // rpass variants {NY, TLR, PAIRS, W, PFR}: TLR threads per row pair (E = NY/TLR
 // points per thread), PAIRS row pairs per workgroup, W minimum waves per SIMD,
 // PFR = 1 loads both rows' G fields up front (shapes with E = 4), PFR = 2 runs
-// one row per TLR threads instead of a pair; the first listed for each NY is
+// one row per TLR threads instead of a pair, PFR = 3 one row with its four G
+// fields loaded before the first transform; the first listed for each NY is
 // the default.  Single rows win at 256 to 1024 (1024: 20.5 against 21.8 us per
 // launch, 512: 10.8 against 13.5 us; these grids run in one round and are
 // bound by a workgroup's chain of transforms), tie at 2048 and lose at 4096
@@ -1210,6 +1229,7 @@ struct KVariant {
   X(256, 64, 1, 1, 1) \
   X(256, 16, 4, 1, 0) \
   X(256, 32, 2, 1, 0) \
+  X(256, 64, 1, 1, 3) \
   X(512, 64, 1, 1, 2) \
   X(512, 64, 1, 1, 0) \
   X(512, 32, 2, 1, 0) \
@@ -1218,6 +1238,7 @@ struct KVariant {
   X(512, 64, 2, 1, 0) \
   X(512, 64, 4, 1, 0) \
   X(512, 128, 1, 1, 2) \
+  X(512, 64, 1, 1, 3) \
   X(1024, 256, 1, 1, 2) \
   X(1024, 256, 1, 1, 0) \
   X(1024, 256, 1, 1, 1) \
@@ -1226,12 +1247,14 @@ struct KVariant {
   X(1024, 64, 2, 1, 0) \
   X(1024, 64, 4, 1, 0) \
   X(1024, 128, 1, 1, 2) \
+  X(1024, 256, 1, 1, 3) \
   X(2048, 128, 1, 2, 0) \
   X(2048, 128, 1, 1, 0) \
   X(2048, 256, 1, 1, 0) \
   X(2048, 512, 1, 1, 0) \
   X(2048, 512, 1, 1, 1) \
   X(2048, 512, 1, 1, 2) \
+  X(2048, 512, 1, 1, 3) \
   X(4096, 512, 1, 4, 0) \
   X(4096, 256, 1, 1, 0) \
   X(4096, 256, 1, 2, 0) \
@@ -1239,7 +1262,8 @@ struct KVariant {
   X(4096, 1024, 1, 1, 0) \
   X(4096, 1024, 1, 1, 1) \
   X(4096, 512, 1, 4, 2) \
-  X(4096, 1024, 1, 1, 2)
+  X(4096, 1024, 1, 1, 2) \
+  X(4096, 1024, 1, 1, 3)
 
 struct RVariant {
   int NY, TLR, PAIRS, W, PFR;

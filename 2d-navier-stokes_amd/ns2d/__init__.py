@@ -86,10 +86,12 @@ def _one_hip_runtime() -> None:
 
 
 def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
-    """Load libns2d_gpu.so (raises OSError when it has not been built)."""
+    """Load libns2d_gpu.so (raises OSError when it has not been built).
+    NS2D_LIB names another build of the library to load instead."""
     global _lib
     if _lib is not None:
         return _lib
+    path = os.environ.get("NS2D_LIB", path)
     if not os.path.exists(path):
         raise OSError(f"{path} not built: run __graft_entry__.build()")
     _one_hip_runtime()

@@ -52,6 +52,21 @@ def alg_bytes(nx: int, ny: int) -> dict:
     return {"rpass": 80.0 * M, "kpass": 124.0 * M, "step": 816.0 * M, "step_model": 928.0 * M}
 
 
+def kernel_signature(pass_name, n):
+    """Template signature of the shape a new solver runs for this pass and
+    length (kpass_kernel<L, C, T, KS, PS, OFF, PK, W, PF> /
+    rpass_kernel<NY, TLR, PAIRS, W, PFR>), as rocprofv3 names the kernel."""
+    import ns2d
+    if pass_name == "kpass":
+        v = ns2d.kpass_variants(n)[ns2d.default_variant(ns2d.PASS_COLUMN, n)]
+        pk = 2 if v["pair_in_registers"] else v["packed_pair"]
+        return (f"kpass_kernel<{n}, {v['cols']}, {v['threads']}, {v['splits']}, {v['pad_shift']}, "
+                f"{v['line_offset']}, {pk}, {v['min_waves']}, {v['prefetch']}>")
+    v = ns2d.rpass_variants(n)[ns2d.default_variant(ns2d.PASS_ROW, n)]
+    pfr = 3 if (v["single_row"] and v["prefetch"]) else 2 if v["single_row"] else v["prefetch"]
+    return f"rpass_kernel<{n}, {v['threads_per_pair']}, {v['pairs']}, {v['min_waves']}, {pfr}>"
+
+
 def make_ic(nx, ny, seed=1):
     from ns2d import ics
     return ics.random_phase(nx, ny, seed=seed, k0=K0, e0=0.5, filt="houli_ref")
@@ -133,16 +148,19 @@ def ensemble_leg(steps=500, seed=2024):
             "ensemble_vs_sequential": t_seq / t_ens}
 
 
-def adjoint_leg(n=512, steps=2000, ckpt=100, seed=5):
+def adjoint_leg(n=512, steps=2000, ckpt=100, seed=5, fd_dirs=3):
     """dZ(T)/dw0 of 2000 RK4CN steps at 512^2 (nu = 1e-4, dt = 1e-3, 2/3,
-    E(k) ~ k^4 exp(-(k/6)^2)), checkpoint every 100 steps with recompute;
-    cost relative to one forward solve of the same length."""
+    E(k) ~ k^4 exp(-(k/6)^2) projected to energy 0.5 and palinstrophy 5000),
+    checkpoint every 100 steps with recompute; cost relative to one forward
+    solve of the same length, and the gradient's agreement with central
+    finite differences of the GPU forward map (Richardson-combined steps) along
+    a few seeded directions."""
     import ns2d
     from ns2d import ics
     nu, dt = 1e-4, 1e-3
     cfg = ns2d.Config(Nx=n, Ny=n, nu=nu, hyper_flag=0, hyper_pow=1.0, alpha=0.0, drag_flag=0, drag_pow=0.0,
                       filter="23", integrator="RK4CN", quirks=0, check_every=0)
-    w0 = ics.random_phase(n, n, seed=seed, k0=6.0, e0=0.5, filt="23")
+    w0 = ics.random_phase_ep(n, n, seed=seed, k0=6.0, e0=0.5, p0=5000.0)
     with ns2d.Solver(cfg) as s:
         s.set_state(w0)
         s.spectral_step(dt, 10)
@@ -153,7 +171,6 @@ def adjoint_leg(n=512, steps=2000, ckpt=100, seed=5):
         s.sync()
         t_fwd = time.perf_counter() - t0
         wT = s.get_state()
-        kx, ky = ics.wavenumbers(n, n)
         c = np.full((1, n // 2 + 1), 2.0)
         c[0, 0] = c[0, -1] = 1.0
         lam = 2.0 * c * wT * (4 * np.pi ** 2) * (0.5 / float(n * n) ** 2)
@@ -162,9 +179,26 @@ def adjoint_leg(n=512, steps=2000, ckpt=100, seed=5):
         t0 = time.perf_counter()
         g = s.vjp(w0, dt, steps, lam, ckpt_every=ckpt)
         t_adj = time.perf_counter() - t0
+
+        def Z(w):
+            s.set_state(w)
+            s.spectral_step(dt, steps)
+            return s.system_measures()["enstrophy"]
+
+        rng = np.random.default_rng(17)
+        amp = np.abs(w0)
+        rels = []
+        for _ in range(fd_dirs):
+            d = amp * np.exp(2j * np.pi * rng.random(w0.shape))
+            h = 1e-4 * np.linalg.norm(w0) / np.linalg.norm(d)
+            D = lambda e: (Z(w0 + e * d) - Z(w0 - e * d)) / (2 * e)
+            fd = (4.0 * D(h / 2) - D(h)) / 3.0
+            an = float(np.sum(g.real * d.real + g.imag * d.imag))
+            rels.append(abs(fd - an) / abs(an))
     return {"adjoint_grid": n, "adjoint_steps": steps, "adjoint_ckpt_every": ckpt,
             "forward_s": t_fwd, "vjp_s": t_adj, "adjoint_over_forward": t_adj / t_fwd,
-            "grad_norm": float(np.linalg.norm(g))}
+            "grad_norm": float(np.linalg.norm(g)), "adjoint_ic": "E = 0.5, P = 5000 (k0 = 6 random phases)",
+            "adjoint_fd_directions": fd_dirs, "adjoint_fd_max_rel_err": max(rels)}
 
 
 def kolmogorov_leg(n=4096, steps=40, warmup=5, seed=3):
@@ -282,11 +316,17 @@ def main():
     else:
         dom, dom_ms, dom_bytes = "rpass", rms, ab["rpass"]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    traffic = None
+    # HBM bytes per launch measured by rocprofv3 --pmc for this exact kernel
+    # shape (profiles/traffic.json, keyed by the kernel's template signature);
+    # null when the shape running here has no such measurement
+    traffic, traffic_src = None, None
+    sig = kernel_signature(dom, nx)
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get(f"{dom}_{nx}")
+            rec = json.load(f).get(sig)
+        if rec:
+            traffic, traffic_src = rec["bytes"], rec["source"]
     extra = {
         "kpass_avg_ms": kms, "rpass_avg_ms": rms,
         "step_alg_GBps": ab["step"] * sps / 1e9,
@@ -326,9 +366,9 @@ def main():
         "config": {"workload": f"decaying turbulence {nx}x{ny} FP64, RK4CN, Hou-Li (integer-division "
                                f"form), nu=5e-5, dt=5e-4, per-step max|w| check",
                    "grid": [nx, ny], "integrator": "RK4CN", "filter": "houli_ref", "parallelism": "1 GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": sig, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
+                     "traffic_source": traffic_src, "alg_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(nx, ny, args.cpu_budget),
         "extra": extra,
     }

@@ -12,3 +12,7 @@ for p in (PKG, os.path.join(ROOT, "oracle"), ROOT):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path through the C ABI)")
+    # the ensemble's tuned-schedule cache of this test session, not the user's
+    import tempfile
+    os.environ.setdefault("NS2D_ENS_CACHE", os.path.join(tempfile.mkdtemp(prefix="ns2d_tests_"),
+                                                         "ens_schedule.jsonl"))

@@ -53,7 +53,7 @@ def test_variant_tables_cover_every_size():
             assert v["per_thread"] * v["threads_per_pair"] == n
             assert v["threads"] == v["threads_per_pair"] * v["pairs"]
             assert v["threads_per_pair"] > 64 and v["pairs"] == 1 or 64 % v["threads_per_pair"] == 0
-            assert not (v["single_row"] and (v["pairs"] != 1 or v["prefetch"])), n
+            assert not (v["single_row"] and v["pairs"] != 1), n
         assert rs[0]["single_row"] == (n in (256, 512, 1024)), n
         if n >= 256:
             assert any(v["single_row"] for v in rs), n

@@ -163,7 +163,7 @@ def test_adaptive_cfl_step_matches_oracle(tmp_path):
     TimeValue attributes and w_hat of every save against the oracle run with
     the same formula."""
     Nx = Ny = 64
-    nu, T, every, cfl = 5e-3, 0.03, 4, 0.5
+    nu, T, every, cfl = 5e-3, 0.4, 4, 0.5
     run(["-o", str(tmp_path) + "/", "-n", str(Nx), "-n", str(Ny), "-s", "0.0", "-e", str(T), "-h", "0.001",
          "-h", "1", "-c", "1", "-c", str(cfl), "-v", str(nu), "-v", "0", "-v", "1.0", "-i", "DECAY_TURB_BB",
          "-p", str(every)], cwd=str(tmp_path))

@@ -105,3 +105,54 @@ def test_graph_replay_matches_direct_launches():
     finally:
         del os.environ["NS2D_GRAPH"]
     assert np.array_equal(a, b)
+
+
+def test_tuned_schedule_cache(monkeypatch, tmp_path):
+    """The tuner's choice per size class is cached as JSON Lines keyed by
+    device, shape tables, class size and member count: a first ensemble tunes
+    and writes one record per class; a later ensemble with the same classes
+    takes its shapes from the cache (here an edited record) without tuning,
+    and still matches plain solves."""
+    import json
+
+    import ns2d
+    cache = tmp_path / "sched.jsonl"
+    monkeypatch.setenv("NS2D_ENS_CACHE", str(cache))
+    monkeypatch.setenv("NS2D_ENS_TUNE", "1")
+    params, w0s, dts = _members()
+    cfgs = [cfg_of(p) for p in params]
+    with ns2d.Ensemble(cfgs) as ens:
+        for i, w in enumerate(w0s):
+            ens[i].set_state(w)
+        ens.step(dts, 2)
+        tuned = [ens.variants(i) for i in range(len(ens))]
+    recs = [json.loads(line) for line in cache.read_text().splitlines()]
+    sizes = sorted({(p.Nx, p.Ny) for p in params})
+    assert sorted((r["nx"], r["ny"]) for r in recs) == sizes
+    for r in recs:
+        i = next(j for j, p in enumerate(params) if (p.Nx, p.Ny) == (r["nx"], r["ny"]))
+        assert (r["kpass"], r["rpass"]) == tuned[i]
+        assert r["members"] == sum(1 for p in params if (p.Nx, p.Ny) == (r["nx"], r["ny"]))
+    # a record naming other (valid) shapes: the next ensemble must use it
+    edited = []
+    for r in recs:
+        r = dict(r, kpass=len(ns2d.kpass_variants(r["nx"])) - 1, rpass=len(ns2d.rpass_variants(r["ny"])) - 1)
+        edited.append(r)
+    with open(cache, "a") as f:
+        for r in edited:
+            f.write(json.dumps(r) + "\n")
+    nsteps = 5
+    with ns2d.Ensemble(cfgs) as ens:
+        for i, w in enumerate(w0s):
+            ens[i].set_state(w)
+        ens.step(dts, nsteps)
+        got = [ens[i].get_state() for i in range(len(ens))]
+        used = [ens.variants(i) for i in range(len(ens))]
+    for i, p in enumerate(params):
+        r = next(r for r in edited if (r["nx"], r["ny"]) == (p.Nx, p.Ny))
+        assert used[i] == (r["kpass"], r["rpass"]), i
+        with ns2d.Solver(cfgs[i]) as s:
+            s.set_state(w0s[i])
+            s.spectral_step(dts[i], nsteps)
+            plain = s.get_state()
+        assert np.linalg.norm(got[i] - plain) / np.linalg.norm(plain) < 1e-13, i

@@ -278,3 +278,18 @@ def test_measures_bins(nx, ny):
     expect = np.ones_like(per_mode)
     expect[0, 0] = 0
     assert np.array_equal(per_mode, expect)
+
+
+@pytest.mark.parametrize("xt", [1, 2, 4, 8])
+def test_spectral_tile_index_is_a_permutation(xt):
+    """spec_idx (ns2d_internal.h): with x-tiles of XT rows, element (x, ky) of
+    an [Nx][ld] array sits at ((x/XT)*ld + ky)*XT + x%XT -- a permutation of
+    [0, Nx*ld) for every grid, so every spectral buffer keeps its size."""
+    s = {1: 0, 2: 1, 4: 2, 8: 3}[xt]
+    for nx in LENGTHS:
+        for ny in LENGTHS:
+            _, ld = ld_of(ny)
+            x = np.arange(nx)[:, None]
+            ky = np.arange(ld)[None, :]
+            idx = (((x >> s) * ld + ky) << s) + (x & (xt - 1))
+            assert np.array_equal(np.sort(idx.ravel()), np.arange(nx * ld)), (nx, ny)
