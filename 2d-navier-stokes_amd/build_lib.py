@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libns2d_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"] + os.environ.get("NS2D_HIPCC_FLAGS", "").split()
-SOURCES = ["ns2d_kernels.hip", "ns2d_adj.hip", "ns2d_capi.cpp"]
+SOURCES = ["ns2d_kernels.hip", "ns2d_adj.hip", "ns2d_noalloc.hip", "ns2d_capi.cpp"]
 PASS_TUS = [(8, 128), (256, 256), (512, 512), (1024, 1024), (2048, 2048), (4096, 4096)]
 DRIVER = os.path.join(HERE, "driver")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")

@@ -915,6 +915,48 @@ int ns2d_measures(ns2d_t* h, int with_flux, ns2d_measures_t* m, double* enrg_spe
   return NS2D_OK;
 }
 
+// ns2d_measures without device allocation: partial sums, spectra and totals
+// in the library's per-device scratch array (ns2d_noalloc.hip), the
+// coefficient table built by ns2d_create.  Results equal ns2d_measures' bit
+// for bit.  Calls that overlap on one GPU share the scratch and race.
+int ns2d_measures_noalloc(ns2d_t* h, int with_flux, ns2d_measures_t* m, double* enrg_spec, double* enst_spec,
+                          double* flux_spec) {
+  if (!h) return fail(NS2D_ERR_INVALID, "null handle");
+  if (h->nspect > NOALLOC_MAX_BINS)
+    return fail(NS2D_ERR_INVALID, "spectrum of %d bins exceeds the allocation-free limit of %d bins (4096 x 4096)",
+                h->nspect, NOALLOC_MAX_BINS);
+  HIPCHK(hipSetDevice(h->p.device));
+  double* scr = nullptr;
+  HIPCHK(noalloc_scratch(h->p.device, &scr));
+  const double* lam = h->lam_tab[visc_kind_plain(h)];
+  if (!lam) return fail(NS2D_ERR_STATE, "coefficient table not built");
+  int rc;
+  if (with_flux) {
+    rc = measures_flux_rhs(h);
+    if (rc) return rc;
+  }
+  const int nb = h->nspect;
+  double* out = scr + (size_t)NQ * NOALLOC_MAX_BINS;
+  const MeasArgs a = measures_args(h, with_flux, lam, scr);
+  hipError_t e = launch_measures(nb, a, h->stream);
+  if (e != hipSuccess) return fail(NS2D_ERR_HIP, "measures launch: %s", hipGetErrorString(e));
+  e = launch_measures_finalize(scr, nb, measures_cn(h), with_flux, out, h->stream);
+  if (e != hipSuccess) return fail(NS2D_ERR_HIP, "measures finalize launch: %s", hipGetErrorString(e));
+  double tot[NQ];
+  HIPCHK(hipMemcpyAsync(tot, out + 8 * (size_t)nb, sizeof tot, hipMemcpyDeviceToHost, h->stream));
+  if (enrg_spec) HIPCHK(hipMemcpyAsync(enrg_spec, out, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream));
+  if (enst_spec) HIPCHK(hipMemcpyAsync(enst_spec, out + nb, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream));
+  if (flux_spec && with_flux)
+    HIPCHK(hipMemcpyAsync(flux_spec, out + 2 * (size_t)nb, sizeof(double) * 6 * nb, hipMemcpyDeviceToHost, h->stream));
+  double2 wf = make_double2(0.0, 0.0);
+  if (h->ph.force_on)
+    HIPCHK(hipMemcpyAsync(&wf, h->w[h->wcur] + spec_idx(0, h->p.force_k, h->ld), sizeof wf, hipMemcpyDeviceToHost,
+                          h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (m) measures_from_totals(h, tot, wf, with_flux, m);
+  return NS2D_OK;
+}
+
 int ns2d_max_vort(ns2d_t* h, double* out) {
   if (!h || !out) return fail(NS2D_ERR_INVALID, "null argument");
   HIPCHK(hipSetDevice(h->p.device));

@@ -164,6 +164,12 @@ hipError_t launch_r2c_rows(int nx, int ny, const double* in, double2* out, const
 hipError_t launch_measures(int nbins, const MeasArgs& a, hipStream_t s);
 hipError_t launch_relayout(const double2* src, double2* dst, int nx, int nyf, int ld, int dir, hipStream_t s);
 
+// allocation-free measures: per-device scratch (ns2d_noalloc.hip) sized for
+// the spectrum length of a 4096 x 4096 grid (sys_msr.c:394)
+constexpr int NOALLOC_MAX_BINS = 2897;
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = (size_t)NOALLOC_MAX_BINS * (NQ + 8) + NQ;
+hipError_t noalloc_scratch(int dev, double** ptr);
+hipError_t launch_measures_finalize(const double* part, int nb, double cn, int with_flux, double* out, hipStream_t s);
 hipError_t launch_kpass_adj(int nx, const KAdjArgs& a, hipStream_t s);
 hipError_t launch_rpass_adj(int nx, int ny, const RAdjArgs& a, hipStream_t s);
 

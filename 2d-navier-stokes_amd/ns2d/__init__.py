@@ -65,7 +65,7 @@ EXPORTS = ["ns2d_params_default", "ns2d_create", "ns2d_destroy", "ns2d_set_state
            "ns2d_max_vort", "ns2d_real_vort", "ns2d_r2c", "ns2d_c2r", "ns2d_get_info", "ns2d_sync",
            "ns2d_get_stream", "ns2d_profile_steps", "ns2d_variant_count", "ns2d_variant_info", "ns2d_variant_extra",
            "ns2d_default_variant", "ns2d_set_variant", "ns2d_ens_create", "ns2d_ens_member", "ns2d_ens_step", "ns2d_ens_autotune", "ns2d_ens_variants", "ns2d_ens_sync",
-           "ns2d_ens_destroy", "ns2d_vjp", "ns2d_vjp_device", "ns2d_last_error"]
+           "ns2d_ens_destroy", "ns2d_vjp", "ns2d_vjp_device", "ns2d_measures_noalloc", "ns2d_last_error"]
 
 _lib = None
 
@@ -109,6 +109,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ns2d_step": ([vp, d, l], i),
         "ns2d_rhs": ([vp, dp, dp], i),
         "ns2d_measures": ([vp, i, P(_Measures), dp, dp, dp], i),
+        "ns2d_measures_noalloc": ([vp, i, P(_Measures), dp, dp, dp], i),
         "ns2d_max_vort": ([vp, dp], i),
         "ns2d_real_vort": ([vp, dp], i),
         "ns2d_r2c": ([vp, dp, dp], i),

@@ -127,6 +127,18 @@ int ns2d_rhs(ns2d_t* h, const double* in, double* out);
 int ns2d_measures(ns2d_t* h, int with_flux, ns2d_measures_t* m, double* enrg_spec, double* enst_spec,
                   double* flux_spec);
 
+/* ns2d_measures for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments and results (bit for bit)
+ * as ns2d_measures; calls no hipMalloc/hipFree: the per-bin partial sums and
+ * spectra live in a file-scope device array of the library (one per GPU,
+ * 510 KB, sized for n_spect <= 2897, i.e. grids up to 4096 x 4096; larger
+ * spectra are rejected with NS2D_ERR_INVALID).  Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU (other threads, other
+ * handles) race. */
+int ns2d_measures_noalloc(ns2d_t* h, int with_flux, ns2d_measures_t* m, double* enrg_spec, double* enst_spec,
+                          double* flux_spec);
+
 /* GetMaxData("VORT") (solver.c:2171-2295): max |c2r(w_hat)| / (Nx*Ny). */
 int ns2d_max_vort(ns2d_t* h, double* out);
 
