@@ -67,7 +67,7 @@ struct ns2d_handle {
   double* meas = nullptr;
   double* wreal = nullptr;
   double* rbuf = nullptr;
-  double2* stage = nullptr;  // [Nx][Ny/2+1] staging between the host and device layouts (x-tiles only)
+  double2* stage = nullptr;  // [Nx][Ny/2+1] staging between the host and the k-space layout (x-tiles only)
   bool inv_valid = false;  // G1..G4 and conj[cur] hold the transforms of the state
   int cur = 0;             // conj[cur] = ky=0 column of the current stage input
   long iters = 0;
@@ -145,7 +145,7 @@ const double* lam_table(ns2d_handle* h, int kind) {
         case VISC_HYPER: v = p.nu * std::pow(k_sqr, p.hyper_pow); break;
         default: v = p.nu * k_sqr; break;
       }
-      t[spec_idx(i, j, ld)] = v;
+      t[kidx(i, j, ld, nx)] = v;
     }
   }
   double* d = nullptr;
@@ -197,7 +197,7 @@ RPassArgs rbase(ns2d_handle* h, int mode) {
 // into / out of the handle's device layout.
 int copy_in(ns2d_handle* h, double2* dst, const void* src, hipMemcpyKind kind) {
   const size_t rowb = h->nyf * sizeof(double2);
-  if constexpr (XT == 1) {
+  if constexpr (XT_K == 1) {
     HIPCHK(hipMemcpy2DAsync(dst, h->ld * sizeof(double2), src, rowb, rowb, h->p.nx, kind, h->stream));
   } else {
     HIPCHK(hipMemcpyAsync(h->stage, src, rowb * h->p.nx, kind, h->stream));
@@ -208,7 +208,7 @@ int copy_in(ns2d_handle* h, double2* dst, const void* src, hipMemcpyKind kind) {
 
 int copy_out(ns2d_handle* h, void* dst, const double2* src, hipMemcpyKind kind) {
   const size_t rowb = h->nyf * sizeof(double2);
-  if constexpr (XT == 1) {
+  if constexpr (XT_K == 1) {
     HIPCHK(hipMemcpy2DAsync(dst, rowb, src, h->ld * sizeof(double2), rowb, h->p.nx, kind, h->stream));
   } else {
     HIPCHK(launch_relayout(src, h->stage, h->p.nx, h->nyf, h->ld, 1, h->stream));
@@ -546,7 +546,7 @@ static int create_impl(const ns2d_params* p, hipStream_t ext_stream, ns2d_t** ou
   }
   if (hipMalloc((void**)&h->pairbuf, sizeof(double2) * 8 * p->nx) != hipSuccess)
     return cleanup(fail(NS2D_ERR_HIP, "hipMalloc failed"));
-  if (XT > 1 && hipMalloc((void**)&h->stage, sizeof(double2) * p->nx * h->nyf) != hipSuccess)
+  if (XT_K != 1 && hipMalloc((void**)&h->stage, sizeof(double2) * p->nx * h->nyf) != hipSuccess)
     return cleanup(fail(NS2D_ERR_HIP, "hipMalloc failed"));
   if (hipMalloc((void**)&h->stat, sizeof(unsigned long long) * STAT_N) != hipSuccess)
     return cleanup(fail(NS2D_ERR_HIP, "hipMalloc failed"));
@@ -569,7 +569,7 @@ static int create_impl(const ns2d_params* p, hipStream_t ext_stream, ns2d_t** ou
     for (int i = 0; i < p->nx; ++i) {
       const long kx = i <= p->nx / 2 ? i : i - p->nx;
       for (int j = 0; j < h->nyf; ++j)
-        t[spec_idx(i, j, h->ld)] = std::exp(-36.0 * std::pow(std::sqrt(std::pow(kx / (p->nx / 2.0), 2.0) +
+        t[kidx(i, j, h->ld, p->nx)] = std::exp(-36.0 * std::pow(std::sqrt(std::pow(kx / (p->nx / 2.0), 2.0) +
                                                                         std::pow(j / (p->ny / 2.0), 2.0)), 36.0));
     }
     if (hipMalloc((void**)&h->filt_tab, sizeof(double) * t.size()) != hipSuccess ||
@@ -890,7 +890,7 @@ int ns2d_measures(ns2d_t* h, int with_flux, ns2d_measures_t* m, double* enrg_spe
   HIPCHK(hipMemcpyAsync(part.data(), h->meas, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));
   double2 wf = make_double2(0.0, 0.0);
   if (h->ph.force_on)
-    HIPCHK(hipMemcpyAsync(&wf, h->w[h->wcur] + spec_idx(0, h->p.force_k, h->ld), sizeof wf, hipMemcpyDeviceToHost,
+    HIPCHK(hipMemcpyAsync(&wf, h->w[h->wcur] + kidx(0, h->p.force_k, h->ld, h->p.nx), sizeof wf, hipMemcpyDeviceToHost,
                           h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   const double cn = measures_cn(h);
@@ -950,7 +950,7 @@ int ns2d_measures_noalloc(ns2d_t* h, int with_flux, ns2d_measures_t* m, double* 
     HIPCHK(hipMemcpyAsync(flux_spec, out + 2 * (size_t)nb, sizeof(double) * 6 * nb, hipMemcpyDeviceToHost, h->stream));
   double2 wf = make_double2(0.0, 0.0);
   if (h->ph.force_on)
-    HIPCHK(hipMemcpyAsync(&wf, h->w[h->wcur] + spec_idx(0, h->p.force_k, h->ld), sizeof wf, hipMemcpyDeviceToHost,
+    HIPCHK(hipMemcpyAsync(&wf, h->w[h->wcur] + kidx(0, h->p.force_k, h->ld, h->p.nx), sizeof wf, hipMemcpyDeviceToHost,
                           h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   if (m) measures_from_totals(h, tot, wf, with_flux, m);

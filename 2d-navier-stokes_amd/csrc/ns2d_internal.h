@@ -4,22 +4,44 @@
 #include <stddef.h>
 
 // Spectral arrays ([Nx][ld] complex, kx rows, ky columns) are stored in
-// x-tiles of NS2D_XTILE rows: element (x, ky) at ((x/XT)*ld + ky)*XT + x%XT,
-// so XT consecutive rows of a column are contiguous (XT = 1: plain row-major).
-#ifndef NS2D_XTILE
-#define NS2D_XTILE 1
+// x-tiles: element (x, ky) at ((x/XT)*ld + ky)*XT + x%XT, so XT consecutive
+// rows of a column are contiguous (XT = 1: plain row-major).  Two layouts:
+// NS2D_XTILE_K for the k-space arrays only the column pass's epilogue (and
+// the measures, the adjoint's k-space recursion and the host copies) touches
+// -- state, accumulator, history, coefficient and filter tables, RHS output;
+// 0 there means column-major, element (x, ky) at ky*Nx + x, so that a column
+// group's columns are whole contiguous runs; NS2D_XTILE_M for the
+// mixed-space fields the row and column passes hand to each other (G1..G4,
+// N, the adjoint's tape and fields).
+// Defaults (measured on MI355X against row-major arrays, per RK4CN step):
+// column-major k-space and 2-row mixed-space tiles, -7.5 % at 1024^2, -6 % at
+// 4096^2 (the column pass -22 %, its 2-column groups reading whole 32-B
+// runs of a tile instead of 16 B per row; the row pass +25 %), -7 % for CN-AB4
+// at 4096^2.
+#ifndef NS2D_XTILE_K
+#define NS2D_XTILE_K 0
+#endif
+#ifndef NS2D_XTILE_M
+#define NS2D_XTILE_M 2
 #endif
 
 namespace ns2d {
 
-constexpr int XT = NS2D_XTILE;
-constexpr int XT_SHIFT = XT == 1 ? 0 : XT == 2 ? 1 : XT == 4 ? 2 : XT == 8 ? 3 : -1;
-static_assert(XT_SHIFT >= 0, "NS2D_XTILE must be 1, 2, 4 or 8");
-
-__host__ __device__ __forceinline__ size_t spec_idx(int x, int ky, int ld) {
+template <int XT>
+__host__ __device__ __forceinline__ size_t tile_idx(int x, int ky, int ld) {
+  // at most 8: the tile height must divide every Nx the ABI accepts
+  static_assert(XT == 1 || XT == 2 || XT == 4 || XT == 8, "tile height must be 1, 2, 4 or 8");
+  constexpr int S = XT == 1 ? 0 : XT == 2 ? 1 : XT == 4 ? 2 : 3;
   if constexpr (XT == 1) return (size_t)x * ld + ky;
-  else return (((size_t)(x >> XT_SHIFT) * ld + ky) << XT_SHIFT) + (size_t)(x & (XT - 1));
+  else return (((size_t)(x >> S) * ld + ky) << S) + (size_t)(x & (XT - 1));
 }
+constexpr int XT_K = NS2D_XTILE_K, XT_M = NS2D_XTILE_M;
+// k-space array index (nx: the grid's Nx) / mixed-space field index
+__host__ __device__ __forceinline__ size_t kidx(int x, int ky, int ld, int nx) {
+  if constexpr (XT_K == 0) return (size_t)ky * nx + x;
+  else return tile_idx<XT_K == 0 ? 1 : XT_K>(x, ky, ld);
+}
+__host__ __device__ __forceinline__ size_t midx(int x, int ky, int ld) { return tile_idx<XT_M>(x, ky, ld); }
 
 // RK4 Butcher tableau and AB4 weights (reference Solver/src/solver.c:29-47)
 constexpr double RK4_A21 = 0.5, RK4_A32 = 0.5, RK4_A43 = 1.0;

@@ -85,15 +85,15 @@ hipError_t launch_kpass_batched(int nx, int nyf, int variant, const KPassArgs* t
 
 // Default column-pass shape: index 0 of its length, except CN-AB4 at 4096.
 // Its epilogue reads three history fields besides the state, and with them
-// 512 threads of 16 rows each beat the 1024-thread shape (600 vs 618 us per
-// launch on MI355X), which the lighter RK4CN stage epilogues prefer
-// (437 vs 451 us).
+// 512 threads of 16 rows each on unpacked columns beat the packed 1024-thread
+// shape (464 vs 475 us per launch on MI355X), which the lighter RK4CN stage
+// epilogues prefer (342 vs 344 us).
 int kpass_default_variant(int nx, bool ab4) {
   if (ab4 && nx == 4096) {
     int n = 0;
     for (const KVariant& v : kvariant_table()) {
       if (v.L != nx) continue;
-      if (v.C == 2 && v.T == 512 && v.KS == 1 && v.PS == 3 && v.OFF == 1 && v.PK == 1 && v.W == 1 && v.PF == 0)
+      if (v.C == 2 && v.T == 512 && v.KS == 1 && v.PS == 3 && v.OFF == 1 && v.PK == 0 && v.W == 1 && v.PF == 0)
         return n;
       ++n;
     }
@@ -237,20 +237,20 @@ hipError_t launch_r2c_rows(int nx, int ny, const double* in, double2* out, const
 }
 
 // Between the host layout [Nx][nyf] (contiguous rows, what callers pass) and
-// the device layout (spec_idx, padding columns ky >= nyf zero): dir 0 packs
+// the device k-space layout (kidx, padding columns ky >= nyf zero): dir 0 packs
 // src (host layout) into dst, dir 1 unpacks src into dst (host layout).
 __global__ __launch_bounds__(256) void relayout_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
-                                                       int nyf, int ld, int dir) {
+                                                       int nyf, int ld, int nx, int dir) {
   const int x = blockIdx.y, ky = blockIdx.x * 256 + threadIdx.x;
   if (dir == 0) {
-    if (ky < ld) dst[spec_idx(x, ky, ld)] = ky < nyf ? src[(size_t)x * nyf + ky] : make_double2(0.0, 0.0);
+    if (ky < ld) dst[kidx(x, ky, ld, nx)] = ky < nyf ? src[(size_t)x * nyf + ky] : make_double2(0.0, 0.0);
   } else if (ky < nyf) {
-    dst[(size_t)x * nyf + ky] = src[spec_idx(x, ky, ld)];
+    dst[(size_t)x * nyf + ky] = src[kidx(x, ky, ld, nx)];
   }
 }
 
 hipError_t launch_relayout(const double2* src, double2* dst, int nx, int nyf, int ld, int dir, hipStream_t s) {
-  hipLaunchKernelGGL(relayout_kernel, dim3((ld + 255) / 256, nx), dim3(256), 0, s, src, dst, nyf, ld, dir);
+  hipLaunchKernelGGL(relayout_kernel, dim3((ld + 255) / 256, nx), dim3(256), 0, s, src, dst, nyf, ld, nx, dir);
   return hipGetLastError();
 }
 

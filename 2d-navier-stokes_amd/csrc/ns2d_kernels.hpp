@@ -38,9 +38,13 @@
 #include <vector>
 
 // NS2D_TW_PRELOAD = 1: each thread loads its FFT twiddles once per kernel
-// (TwReg) instead of from the table inside every transform pass.
+// (TwReg) instead of from the table inside every transform pass.  Off by
+// default: the registers the preloaded set holds cost occupancy (and spills
+// in the 1024-thread shapes), and MI355X ran the passes slower with it
+// (1024^2: column pass 36.2 against 33.3 us, 4096^2 RK4CN 357 against 384
+// steps/s).
 #ifndef NS2D_TW_PRELOAD
-#define NS2D_TW_PRELOAD 1
+#define NS2D_TW_PRELOAD 0
 #endif
 
 namespace ns2d {
@@ -84,7 +88,7 @@ __device__ __forceinline__ double dealias_factor(const PhysArgs& ph, int kx, int
       return q == 0 ? 1.0 : (q == 1 ? ph.houli_edge : ph.houli_corner);
     }
     case FILTER_HOULI:   // exp(-36 (|k|/k_N)^36), tabulated on the host
-      return ph.filt_tab[spec_idx(kx < 0 ? kx + ph.nx : kx, ky, ph.ld)];
+      return ph.filt_tab[kidx(kx < 0 ? kx + ph.nx : kx, ky, ph.ld, ph.nx)];
     default: return 1.0;
   }
 }
@@ -184,7 +188,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
       for (int it = 0; it < E; ++it) {
         const int i = l + TL * it;
         const int kx = kx_of_row(i, nx);
-        const size_t idx = spec_idx(i, ky, ld);
+        const size_t idx = kidx(i, ky, ld, nx);
         wpre[it] = !pre ? make_double2(0.0, 0.0) : (cf && kx < 0) ? cconj(a.conj_src[-kx]) : a.w_in[idx];
         apre[it] = (pre && need_acc) ? a.acc[idx] : make_double2(0.0, 0.0);
         lpre[it] = (pre && need_lam) ? a.lam[idx] : 0.0;
@@ -197,8 +201,8 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int x = l + TL * e;
-      if (pair) R[e] = make_double2(a.nin[spec_idx(x, 0, ld)].x, a.nin[spec_idx(x, H, ld)].x);
-      else R[e] = colv ? a.nin[spec_idx(x, ky, ld)] : make_double2(0.0, 0.0);
+      if (pair) R[e] = make_double2(a.nin[midx(x, 0, ld)].x, a.nin[midx(x, H, ld)].x);
+      else R[e] = colv ? a.nin[midx(x, ky, ld)] : make_double2(0.0, 0.0);
       if ((e & 3) == 3) asm volatile("" ::: "memory");
     }
     fft_reg<L, TL, -1, BLOCK, PS>(R, buf, tw, opaque(l));
@@ -233,7 +237,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
     // held live across it
     const int kq = opaque(q ? H : ky);
     const int lq = opaque(l);
-    auto index_of = [&](int it) -> size_t { return spec_idx(lq + TL * it, kq, ld); };
+    auto index_of = [&](int it) -> size_t { return kidx(lq + TL * it, kq, ld, nx); };
     if (PKR && q == 1) {
       // S_0 to RH, KH (forward modes) to R
 #pragma unroll
@@ -275,7 +279,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
       const int i = lq + TL * it;
       const int kx = kx_of_row(i, nx);
       if (conj_fix && kx < 0) return cconj(a.conj_src[-kx]);
-      return a.w_in[spec_idx(i, kq, ld)];
+      return a.w_in[kidx(i, kq, ld, nx)];
     };
     auto acc_at = [&](int it) -> double2 {
       if constexpr (PF != 0) return apre[it];
@@ -468,7 +472,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
       double* __restrict__ g4 = reinterpret_cast<double*>(a.g[3]);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const size_t r0 = spec_idx(l_o + TL * e, 0, ld), rh = spec_idx(l_o + TL * e, H, ld);
+        const size_t r0 = midx(l_o + TL * e, 0, ld), rh = midx(l_o + TL * e, H, ld);
         if (f == 0) {
           gout[r0] = make_double2(0.0, 0.0);
           gout[rh] = make_double2(0.0, F[e].x);
@@ -483,7 +487,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
       }
     } else if (colv) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) gout[spec_idx(l_o + TL * e, ky_o, ld)] = F[e];
+      for (int e = 0; e < E; ++e) gout[midx(l_o + TL * e, ky_o, ld)] = F[e];
     }
   }
 }
@@ -557,18 +561,18 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
       double2 A, B;
       const double fk = (double)kk;
       if (which == 0) {         // u = i ky G1, v = G2
-        const double2 g1 = a.g[0][spec_idx(x, kk, ld)];
+        const double2 g1 = a.g[0][midx(x, kk, ld)];
         A = make_double2(-fk * g1.y, fk * g1.x);
-        B = a.g[1][spec_idx(x, kk, ld)];
+        B = a.g[1][midx(x, kk, ld)];
       } else if (which == 1) {  // w_x = G3, w_y = i ky G4
-        A = a.g[2][spec_idx(x, kk, ld)];
-        const double2 g4 = a.g[3][spec_idx(x, kk, ld)];
+        A = a.g[2][midx(x, kk, ld)];
+        const double2 g4 = a.g[3][midx(x, kk, ld)];
         B = make_double2(-fk * g4.y, fk * g4.x);
       } else if (which == 2) {  // w of rows x and x+1 = G4
-        A = a.g[3][spec_idx(x, kk, ld)];
-        B = a.g[3][spec_idx(x + 1, kk, ld)];
+        A = a.g[3][midx(x, kk, ld)];
+        B = a.g[3][midx(x + 1, kk, ld)];
       } else {                  // w of row x alone
-        A = a.g[3][spec_idx(x, kk, ld)];
+        A = a.g[3][midx(x, kk, ld)];
         B = make_double2(0.0, 0.0);
       }
       if (kk == 0 || kk == H) { A.y = 0.0; B.y = 0.0; }
@@ -590,7 +594,7 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
           for (int e = 0; e < E; ++e) {
             const int ky = l + TLR * e;
             const int kk = ky > H ? NY - ky : ky;
-            pre[r][f][e] = a.g[f][spec_idx(x0 + r, kk, ld)];
+            pre[r][f][e] = a.g[f][midx(x0 + r, kk, ld)];
           }
       // the same assembly as build() from the loaded values
       auto build_pre = [&](double2* z, int which, const double2 (&g)[4][E]) {
@@ -697,7 +701,7 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = l + TLR * e;
-      if (k <= H) a.nout[spec_idx(x0, k, ld)] = (k == 0 || k == H) ? make_double2(w[e].x, 0.0) : w[e];
+      if (k <= H) a.nout[midx(x0, k, ld)] = (k == 0 || k == H) ? make_double2(w[e].x, 0.0) : w[e];
     }
   } else if ((a.mode & RP_NL) && (BLOCK || valid)) {
     fft_reg<NY, TLR, -1, BLOCK>(w, buf, tw, opaque(l));
@@ -711,23 +715,26 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
       if (k <= H) {
         const double2 p = w[e];
         const double2 qm = cconj(buf[padx((NY - k) & (NY - 1))]);
-        a.nout[spec_idx(x0, k, ld)] = make_double2(0.5 * (p.x + qm.x), 0.5 * (p.y + qm.y));
-        a.nout[spec_idx(x0 + 1, k, ld)] = make_double2(0.5 * (p.y - qm.y), -0.5 * (p.x - qm.x));
+        a.nout[midx(x0, k, ld)] = make_double2(0.5 * (p.x + qm.x), 0.5 * (p.y + qm.y));
+        a.nout[midx(x0 + 1, k, ld)] = make_double2(0.5 * (p.y - qm.y), -0.5 * (p.x - qm.x));
       }
     }
   }
 }
 
 // W: minimum waves per SIMD the register allocation must allow (1 = no limit)
+// Consecutive rows (row pairs) go to one XCD, whose L2 then serves the
+// x-tiles of the mixed-space fields that neighbouring rows share.
 template <int NY, int TLR, int PAIRS, int W, int PFR>
 __global__ __launch_bounds__(TLR * PAIRS, W) void rpass_kernel(RPassArgs a) {
-  rpass_body<NY, TLR, PAIRS, PFR>(a, (int)blockIdx.x);
+  rpass_body<NY, TLR, PAIRS, PFR>(a, xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
 template <int NY, int TLR, int PAIRS, int W, int PFR>
 __global__ __launch_bounds__(TLR * PAIRS, W) void rpass_batched(const RPassArgs* __restrict__ tab, int per_member) {
-  const int m = (int)blockIdx.x / per_member;
-  rpass_body<NY, TLR, PAIRS, PFR>(tab[m], (int)blockIdx.x - m * per_member);
+  const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int m = b / per_member;
+  rpass_body<NY, TLR, PAIRS, PFR>(tab[m], b - m * per_member);
 }
 
 // ---------------------------------------------------------------------------
@@ -779,7 +786,7 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
       const bool mirror = ky > H;
       const int kk = mirror ? NY - ky : ky;
       const double fk = (double)kk;
-      double2 A = fa[spec_idx(xa, kk, ld)], B = hasb ? fb[spec_idx(xb, kk, ld)] : make_double2(0.0, 0.0);
+      double2 A = fa[midx(xa, kk, ld)], B = hasb ? fb[midx(xb, kk, ld)] : make_double2(0.0, 0.0);
       if (mula) A = make_double2(-fk * A.y, fk * A.x);   // times i ky
       if (mulb) B = make_double2(-fk * B.y, fk * B.x);
       if (kk == 0 || kk == H) { A.y = 0.0; B.y = 0.0; }
@@ -806,8 +813,8 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
         const double fk = (double)k;
         if (mulp) P = make_double2(fk * P.y, -fk * P.x);   // times -i k
         if (mulq) Q = make_double2(fk * Q.y, -fk * Q.x);
-        pout[spec_idx(x, k, ld)] = cscale(w2, P);
-        qout[spec_idx(x, k, ld)] = cscale(w2, Q);
+        pout[midx(x, k, ld)] = cscale(w2, P);
+        qout[midx(x, k, ld)] = cscale(w2, Q);
       }
     }
     xsync<BLOCK>();
@@ -842,7 +849,7 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
 
 template <int NY, int TLR, int PAIRS, int SR>
 __global__ __launch_bounds__(TLR * PAIRS) void rpass_adj_kernel(RAdjArgs a) {
-  rpass_adj_body<NY, TLR, PAIRS, SR>(a, (int)blockIdx.x);
+  rpass_adj_body<NY, TLR, PAIRS, SR>(a, xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
 template <int L, int C, int T>
@@ -876,7 +883,7 @@ __global__ __launch_bounds__(T) void kpass_adj_kernel(KAdjArgs a) {
       double2 F[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        F[e] = colv ? a.bin[f][spec_idx(l_o + TL * e, ky_o, ld)] : make_double2(0.0, 0.0);
+        F[e] = colv ? a.bin[f][midx(l_o + TL * e, ky_o, ld)] : make_double2(0.0, 0.0);
         if ((e & 3) == 3) asm volatile("" ::: "memory");
       }
       fft_reg<L, TL, -1, BLOCK>(F, buf, tw, l_o);
@@ -903,7 +910,7 @@ __global__ __launch_bounds__(T) void kpass_adj_kernel(KAdjArgs a) {
     LK[e] = make_double2(0.0, 0.0);
     if (!colv) continue;
     const int kx = kx_of_row(l + TL * e, nx);
-    const size_t idx = spec_idx(l + TL * e, ky, ld);
+    const size_t idx = kidx(l + TL * e, ky, ld, nx);
     const bool zero = (kx == 0 && ky == 0);
     const double D = h * a.lam[idx];
     const double A = (2.0 - D) / (2.0 + D), B = 2.0 * h / (2.0 + D);
@@ -958,7 +965,7 @@ __global__ __launch_bounds__(T) void kpass_adj_kernel(KAdjArgs a) {
     fft_reg<L, TL, +1, BLOCK>(LK, buf, tw, l_o);
     if (colv) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) a.lout[spec_idx(l_o + TL * e, ky_o, ld)] = LK[e];
+      for (int e = 0; e < E; ++e) a.lout[midx(l_o + TL * e, ky_o, ld)] = LK[e];
     }
   }
 }
@@ -998,7 +1005,7 @@ __global__ __launch_bounds__(T) void r2c_rows_kernel(const double* __restrict__ 
         const double2 O = make_double2(0.5 * (p.y - q.y), -0.5 * (p.x - q.x));
         X = cadd(E, cmul(tw[k * TWS], O));
       }
-      out[spec_idx(x, k, ld)] = X;
+      out[midx(x, k, ld)] = X;
     }
   }
 }
@@ -1036,7 +1043,7 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
       if (yhi > H) yhi = H;
       const int i = kx >= 0 ? kx : kx + nx;
       for (long ky = ylo; ky <= yhi; ++ky) {
-        const size_t idx = spec_idx(i, (int)ky, a.ld);
+        const size_t idx = kidx(i, (int)ky, a.ld, nx);
         const double2 w = a.w_hat[idx];
         const double k2 = (double)(kx2 + ky * ky);
         const double c = (ky == 0 || ky == H) ? 1.0 : 2.0;
@@ -1210,10 +1217,11 @@ struct KVariant {
 // PFR = 1 loads both rows' G fields up front (shapes with E = 4), PFR = 2 runs
 // one row per TLR threads instead of a pair, PFR = 3 one row with its four G
 // fields loaded before the first transform; the first listed for each NY is
-// the default.  Single rows win at 256 to 1024 (1024: 20.5 against 21.8 us per
-// launch, 512: 10.8 against 13.5 us; these grids run in one round and are
-// bound by a workgroup's chain of transforms), tie at 2048 and lose at 4096
-// (224 against 205 us: four rounds, where the extra transform counts).
+// the default.  Single rows win at 256 to 1024 (these grids run in one round
+// and are bound by a workgroup's chain of transforms) and at 4096, where the
+// mixed-space fields' 2-row tiles make a row pair's sequential halves read
+// each 32-B run twice (single rows 260 against 287 us per launch, the two
+// rows of a tile on one XCD); at 2048 256-thread row pairs (64 against 74 us).
 #define NS2D_RPASS_VARIANTS(X) \
   X(8, 1, 64, 1, 0) \
   X(16, 1, 64, 1, 0) \
@@ -1248,20 +1256,20 @@ struct KVariant {
   X(1024, 64, 4, 1, 0) \
   X(1024, 128, 1, 1, 2) \
   X(1024, 256, 1, 1, 3) \
+  X(2048, 256, 1, 1, 0) \
   X(2048, 128, 1, 2, 0) \
   X(2048, 128, 1, 1, 0) \
-  X(2048, 256, 1, 1, 0) \
   X(2048, 512, 1, 1, 0) \
   X(2048, 512, 1, 1, 1) \
   X(2048, 512, 1, 1, 2) \
   X(2048, 512, 1, 1, 3) \
+  X(4096, 512, 1, 4, 2) \
   X(4096, 512, 1, 4, 0) \
   X(4096, 256, 1, 1, 0) \
   X(4096, 256, 1, 2, 0) \
   X(4096, 512, 1, 1, 0) \
   X(4096, 1024, 1, 1, 0) \
   X(4096, 1024, 1, 1, 1) \
-  X(4096, 512, 1, 4, 2) \
   X(4096, 1024, 1, 1, 2) \
   X(4096, 1024, 1, 1, 3)
 

@@ -42,7 +42,7 @@ def build(verbose: bool = False) -> str:
         text = re.sub(r'asm volatile\(""[^;]*\);', ";", text)
         open(os.path.join(src, f), "w").write(text)
     cxx = ["g++", "-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address", "-fPIC",
-           "-I" + HERE, "-I" + src, "-w"]
+           "-I" + HERE, "-I" + src, "-w"] + os.environ.get("NS2D_EMU_FLAGS", "").split()
     jobs = [cxx + ["-c", os.path.join(HERE, "emu_rt.cpp"), "-o", os.path.join(obj, "emu_rt.o")]]
     for s in ("ns2d_kernels.hip", "ns2d_adj.hip", "ns2d_noalloc.hip", "ns2d_capi.cpp"):
         if not os.path.exists(os.path.join(src, s)):

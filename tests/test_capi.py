@@ -38,12 +38,12 @@ def test_variant_tables_cover_every_size():
         assert vs[0]["packed_pair"] == (1 if n == 4096 else 0), n
         assert all(v["min_waves"] >= 1 and v["pad_shift"] >= 3 for v in vs), n
         # defaults: index 0 of each length, except CN-AB4's column pass at
-        # 4096 (a packed 512-thread shape; RK4CN's is the 1024-thread one)
+        # 4096 (an unpacked 512-thread shape; RK4CN's is the packed 1024-thread one)
         assert ns2d.default_variant(ns2d.PASS_ROW, n, "AB4") == 0
         assert ns2d.default_variant(ns2d.PASS_COLUMN, n, "RK4CN") == 0
         ab4 = ns2d.default_variant(ns2d.PASS_COLUMN, n, "AB4")
         if n == 4096:
-            assert ab4 > 0 and vs[ab4]["threads"] == 512 and vs[ab4]["cols"] == 2 and vs[ab4]["packed_pair"]
+            assert ab4 > 0 and vs[ab4]["threads"] == 512 and vs[ab4]["cols"] == 2 and not vs[ab4]["packed_pair"]
             assert vs[ab4]["splits"] == 1 and not vs[ab4]["prefetch"] and vs[0]["threads"] == 1024
         else:
             assert ab4 == 0, n
@@ -54,7 +54,7 @@ def test_variant_tables_cover_every_size():
             assert v["threads"] == v["threads_per_pair"] * v["pairs"]
             assert v["threads_per_pair"] > 64 and v["pairs"] == 1 or 64 % v["threads_per_pair"] == 0
             assert not (v["single_row"] and v["pairs"] != 1), n
-        assert rs[0]["single_row"] == (n in (256, 512, 1024)), n
+        assert rs[0]["single_row"] == (n in (256, 512, 1024, 4096)), n
         if n >= 256:
             assert any(v["single_row"] for v in rs), n
 

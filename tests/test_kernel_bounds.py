@@ -282,9 +282,10 @@ def test_measures_bins(nx, ny):
 
 @pytest.mark.parametrize("xt", [1, 2, 4, 8])
 def test_spectral_tile_index_is_a_permutation(xt):
-    """spec_idx (ns2d_internal.h): with x-tiles of XT rows, element (x, ky) of
-    an [Nx][ld] array sits at ((x/XT)*ld + ky)*XT + x%XT -- a permutation of
-    [0, Nx*ld) for every grid, so every spectral buffer keeps its size."""
+    """tile_idx (ns2d_internal.h, the kidx / midx layouts): with x-tiles of XT
+    rows, element (x, ky) of an [Nx][ld] array sits at ((x/XT)*ld + ky)*XT +
+    x%XT -- a permutation of [0, Nx*ld) for every grid the ABI accepts (XT <= 8
+    divides every Nx), so every spectral buffer keeps its size."""
     s = {1: 0, 2: 1, 4: 2, 8: 3}[xt]
     for nx in LENGTHS:
         for ny in LENGTHS:
