@@ -413,6 +413,13 @@ int ns2d_variant_extra(int pass, int n, int idx, int* extra4) {
   return NS2D_OK;
 }
 
+int ns2d_layout(int* xtile_k, int* xtile_m) {
+  if (!xtile_k || !xtile_m) return fail(NS2D_ERR_INVALID, "null argument");
+  *xtile_k = XT_K;
+  *xtile_m = XT_M;
+  return NS2D_OK;
+}
+
 int ns2d_default_variant(int pass, int n, int integrator, int* idx) {
   if (!idx) return fail(NS2D_ERR_INVALID, "null argument");
   if (integrator != NS2D_INT_RK4CN && integrator != NS2D_INT_AB4)
@@ -821,6 +828,7 @@ MeasArgs measures_args(ns2d_handle* h, int with_flux, const double* lam, double*
   a.ld = h->ld;
   a.lam = lam;
   a.quirk_enst_diss = (h->p.quirks & NS2D_QUIRK_ENST_DISS) ? 1 : 0;
+  a.nbins = h->nspect;
   a.sbst_lo = 0.0;                   // LWR_SBST_LIM^2, data_types.h:175
   a.sbst_hi = 64.0 * 64.0;           // UPR_SBST_LIM^2, data_types.h:174
   a.ph = h->ph;

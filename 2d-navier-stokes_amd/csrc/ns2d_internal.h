@@ -155,6 +155,7 @@ struct MeasArgs {
   double* out;
   const double* lam;     // per-mode pre_fac (sys_msr.c:160-175), [Nx][ld]
   int nx, ny, ld, quirk_enst_diss;
+  int nbins;             // spectrum bins (workgroups past the last one exit)
   double sbst_lo, sbst_hi;
   PhysArgs ph;
 };
@@ -168,6 +169,18 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
   const int xcd = b % NXCD, j = b / NXCD;
   return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + j;
 }
+
+// Spectrum bin of measures workgroup g: chunks of MEAS_CHUNK consecutive bins
+// dealt round-robin to the 8 XCDs (workgroup g runs on XCD g % 8), so that the
+// bins sharing cache lines (neighbouring annuli) run on one XCD's L2 while
+// every XCD gets inner and outer bins alike.  The grid is rounded up to
+// 8 * MEAS_CHUNK workgroups; g maps onto [0, grid) one to one.
+constexpr int MEAS_CHUNK = 32;
+__host__ __device__ __forceinline__ int meas_bin(int g) {
+  const int xcd = g % 8, j = g / 8;
+  return ((j / MEAS_CHUNK) * 8 + xcd) * MEAS_CHUNK + j % MEAS_CHUNK;
+}
+inline int meas_grid(int nbins) { return (nbins + 8 * MEAS_CHUNK - 1) / (8 * MEAS_CHUNK) * (8 * MEAS_CHUNK); }
 
 hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s);
 int kpass_variant_count(int nx);

@@ -255,7 +255,8 @@ hipError_t launch_relayout(const double2* src, double2* dst, int nx, int nyf, in
 }
 
 hipError_t launch_measures(int nbins, const MeasArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((measures_kernel<256>), dim3(nbins), dim3(256), 0, s, a);
+  if (a.nbins != nbins) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((measures_kernel<256>), dim3(meas_grid(nbins)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

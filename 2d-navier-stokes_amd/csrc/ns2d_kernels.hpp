@@ -1024,7 +1024,8 @@ __device__ __forceinline__ long isqrt_floor(long v) {
 template <int T>
 __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   __shared__ double red[NQ][T / 64];
-  const int b = blockIdx.x;
+  const int b = meas_bin((int)blockIdx.x);
+  if (b >= a.nbins) return;
   const int tid = threadIdx.x;
   const int nx = a.nx, ny = a.ny, H = ny / 2;
   double acc[NQ];

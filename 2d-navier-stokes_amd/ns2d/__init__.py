@@ -65,7 +65,7 @@ EXPORTS = ["ns2d_params_default", "ns2d_create", "ns2d_destroy", "ns2d_set_state
            "ns2d_max_vort", "ns2d_real_vort", "ns2d_r2c", "ns2d_c2r", "ns2d_get_info", "ns2d_sync",
            "ns2d_get_stream", "ns2d_profile_steps", "ns2d_variant_count", "ns2d_variant_info", "ns2d_variant_extra",
            "ns2d_default_variant", "ns2d_set_variant", "ns2d_ens_create", "ns2d_ens_member", "ns2d_ens_step", "ns2d_ens_autotune", "ns2d_ens_variants", "ns2d_ens_sync",
-           "ns2d_ens_destroy", "ns2d_vjp", "ns2d_vjp_device", "ns2d_measures_noalloc", "ns2d_last_error"]
+           "ns2d_ens_destroy", "ns2d_vjp", "ns2d_vjp_device", "ns2d_measures_noalloc", "ns2d_layout", "ns2d_last_error"]
 
 _lib = None
 
@@ -123,6 +123,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ns2d_variant_extra": ([i, i, i, P(i)], i),
         "ns2d_default_variant": ([i, i, i, P(i)], i),
         "ns2d_set_variant": ([vp, i, i], i),
+        "ns2d_layout": ([P(i), P(i)], i),
         "ns2d_ens_create": ([P(_Params), i, P(vp)], i),
         "ns2d_ens_member": ([vp, i, P(vp)], i),
         "ns2d_ens_step": ([vp, dp, l], i),
@@ -180,6 +181,15 @@ def default_variant(pass_kind: int, n: int, integrator: str = "RK4CN") -> int:
     idx = ctypes.c_int()
     _check(lib.ns2d_default_variant(pass_kind, n, INTEGRATORS[integrator], ctypes.byref(idx)))
     return idx.value
+
+
+def layout() -> tuple:
+    """(k-space x-tile, mixed-space x-tile) the library stores its device
+    arrays in; k-space 0 = column-major [Ny/2+1][Nx]."""
+    lib = load_library()
+    k, m = ctypes.c_int(), ctypes.c_int()
+    _check(lib.ns2d_layout(ctypes.byref(k), ctypes.byref(m)))
+    return k.value, m.value
 
 
 class NS2DError(RuntimeError):

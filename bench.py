@@ -67,6 +67,14 @@ def kernel_signature(pass_name, n):
     return f"rpass_kernel<{n}, {v['threads_per_pair']}, {v['pairs']}, {v['min_waves']}, {pfr}>"
 
 
+def traffic_key(pass_name, n):
+    """kernel_signature plus the device layout the library was compiled with
+    (k-space x-tile K, 0 = column-major; mixed-space x-tile M): "<sig> layout kKmM"."""
+    import ns2d
+    k, m = ns2d.layout()
+    return f"{kernel_signature(pass_name, n)} layout k{k}m{m}"
+
+
 def make_ic(nx, ny, seed=1):
     from ns2d import ics
     return ics.random_phase(nx, ny, seed=seed, k0=K0, e0=0.5, filt="houli_ref")
@@ -201,6 +209,39 @@ def adjoint_leg(n=512, steps=2000, ckpt=100, seed=5, fd_dirs=3):
             "adjoint_fd_directions": fd_dirs, "adjoint_fd_max_rel_err": max(rels)}
 
 
+def aux_profile():
+    """The adjoint and measures kernels' workloads for a kernel trace: one
+    vector-Jacobian product of 400 RK4CN steps at 512^2 (the adjoint leg's
+    flow and initial condition, checkpoints every 100 steps) and 10 measure
+    calls with the flux terms at 1024^2 (the bench flow's initial state)."""
+    import ns2d
+    from ns2d import ics
+    n, dt, steps = 512, 1e-3, 400
+    cfg = ns2d.Config(Nx=n, Ny=n, nu=1e-4, hyper_flag=0, hyper_pow=1.0, alpha=0.0, drag_flag=0, drag_pow=0.0,
+                      filter="23", integrator="RK4CN", quirks=0, check_every=0)
+    w0 = ics.random_phase_ep(n, n, seed=5, k0=6.0, e0=0.5, p0=5000.0)
+    with ns2d.Solver(cfg) as s:
+        s.set_state(w0)
+        s.spectral_step(dt, steps)
+        wT = s.get_state()
+        c = np.full((1, n // 2 + 1), 2.0)
+        c[0, 0] = c[0, -1] = 1.0
+        lam = 2.0 * c * wT * (4 * np.pi ** 2) * (0.5 / float(n * n) ** 2)
+        lam[0, 0] = 0.0
+        t0 = time.perf_counter()
+        g = s.vjp(w0, dt, steps, lam, ckpt_every=100)
+        print(f"adjoint {n}^2 x {steps} steps: {time.perf_counter() - t0:.3f} s, |g| = "
+              f"{float(np.linalg.norm(g)):.6e}", flush=True)
+    N = 1024
+    with make_solver(N, N) as s:
+        s.set_state(make_ic(N, N))
+        t0 = time.perf_counter()
+        for _ in range(10):
+            m = s.system_measures(with_flux=True)
+        print(f"measures {N}^2 with flux x 10: {time.perf_counter() - t0:.3f} s, E = {m['energy']:.12e}",
+              flush=True)
+
+
 def kolmogorov_leg(n=4096, steps=40, warmup=5, seed=3):
     """Forced Kolmogorov flow at 4096^2: f = F sin(k_f y), k_f = 4, F = 1,
     linear drag alpha = 0.01, nu = 2e-5, 2/3 filter, CN-AB4 after 3 RK4CN
@@ -302,11 +343,18 @@ def main():
                     help="end with a 1 GiB device-to-device copy (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--aux-profile", action="store_true",
+                    help="run only the adjoint (512^2, 400 steps, checkpoints every 100) and the "
+                         "flux measures (1024^2, 10 calls) workloads, for a kernel trace; prints no "
+                         "bench line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU (slab-decomposed) runs are not "
               "part of it -- use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.aux_profile:
+        aux_profile()
+        return
     nx = ny = args.n
     el, kms, rms, meas = time_gpu(nx, ny, args.steps, args.warmup)
     sps = args.steps / el
@@ -317,10 +365,11 @@ def main():
         dom, dom_ms, dom_bytes = "rpass", rms, ab["rpass"]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     # HBM bytes per launch measured by rocprofv3 --pmc for this exact kernel
-    # shape (profiles/traffic.json, keyed by the kernel's template signature);
-    # null when the shape running here has no such measurement
+    # shape and device layout (profiles/traffic.json, keyed by the kernel's
+    # template signature and the library's layout); null when the shape
+    # running here has no such measurement
     traffic, traffic_src = None, None
-    sig = kernel_signature(dom, nx)
+    sig = traffic_key(dom, nx)
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:

@@ -280,6 +280,30 @@ def test_measures_bins(nx, ny):
     assert np.array_equal(per_mode, expect)
 
 
+@pytest.mark.parametrize("nbins", [1, 5, 8, 255, 256, 257, 725, 2897, 5793])
+def test_measures_workgroup_to_bin_map(nbins):
+    """meas_bin / meas_grid (ns2d_internal.h): the measures grid is rounded up
+    to 8 * MEAS_CHUNK workgroups and workgroup g computes bin
+    ((j / CH) * 8 + g % 8) * CH + j % CH with j = g / 8 -- one to one on
+    [0, grid), so every bin b < n_spect is summed by exactly one workgroup
+    (the rest exit), consecutive bins of a chunk share an XCD (g % 8), and the
+    XCDs' shares of bins differ by at most one chunk."""
+    CH = 32
+    grid = -(-nbins // (8 * CH)) * (8 * CH)
+    g = np.arange(grid)
+    j = g // 8
+    b = ((j // CH) * 8 + g % 8) * CH + j % CH
+    assert np.array_equal(np.sort(b), np.arange(grid))
+    live = b < nbins
+    assert np.array_equal(np.sort(b[live]), np.arange(nbins))
+    xcd_of_bin = np.empty(grid, dtype=np.int64)
+    xcd_of_bin[b] = g % 8
+    chunks = xcd_of_bin.reshape(-1, CH)
+    assert np.all(chunks == chunks[:, :1])
+    share = np.bincount(xcd_of_bin[:nbins], minlength=8)
+    assert share.max() - share.min() <= CH
+
+
 @pytest.mark.parametrize("xt", [1, 2, 4, 8])
 def test_spectral_tile_index_is_a_permutation(xt):
     """tile_idx (ns2d_internal.h, the kidx / midx layouts): with x-tiles of XT
