@@ -413,10 +413,11 @@ int ns2d_variant_extra(int pass, int n, int idx, int* extra4) {
   return NS2D_OK;
 }
 
-int ns2d_layout(int* xtile_k, int* xtile_m) {
-  if (!xtile_k || !xtile_m) return fail(NS2D_ERR_INVALID, "null argument");
+int ns2d_layout(int* xtile_k, int* xtile_g, int* xtile_n) {
+  if (!xtile_k || !xtile_g || !xtile_n) return fail(NS2D_ERR_INVALID, "null argument");
   *xtile_k = XT_K;
-  *xtile_m = XT_M;
+  *xtile_g = XT_G;
+  *xtile_n = XT_N;
   return NS2D_OK;
 }
 

@@ -21,8 +21,19 @@
 #ifndef NS2D_XTILE_K
 #define NS2D_XTILE_K 0
 #endif
+// The mixed-space fields go two ways: G (the column pass's four inverse
+// fields, and the adjoint's l_n transform) from the column pass to the row
+// pass, N (the row pass's output, the adjoint's B fields, the initial
+// conditions' row transforms) from the row pass to the column pass; each set
+// has its own tile height.
 #ifndef NS2D_XTILE_M
 #define NS2D_XTILE_M 2
+#endif
+#ifndef NS2D_XTILE_G
+#define NS2D_XTILE_G NS2D_XTILE_M
+#endif
+#ifndef NS2D_XTILE_N
+#define NS2D_XTILE_N NS2D_XTILE_M
 #endif
 
 namespace ns2d {
@@ -35,13 +46,16 @@ __host__ __device__ __forceinline__ size_t tile_idx(int x, int ky, int ld) {
   if constexpr (XT == 1) return (size_t)x * ld + ky;
   else return (((size_t)(x >> S) * ld + ky) << S) + (size_t)(x & (XT - 1));
 }
-constexpr int XT_K = NS2D_XTILE_K, XT_M = NS2D_XTILE_M;
-// k-space array index (nx: the grid's Nx) / mixed-space field index
+constexpr int XT_K = NS2D_XTILE_K, XT_G = NS2D_XTILE_G, XT_N = NS2D_XTILE_N;
+// k-space array index (nx: the grid's Nx)
 __host__ __device__ __forceinline__ size_t kidx(int x, int ky, int ld, int nx) {
   if constexpr (XT_K == 0) return (size_t)ky * nx + x;
   else return tile_idx<XT_K == 0 ? 1 : XT_K>(x, ky, ld);
 }
-__host__ __device__ __forceinline__ size_t midx(int x, int ky, int ld) { return tile_idx<XT_M>(x, ky, ld); }
+// mixed-space index of a field the column pass writes and the row pass reads
+__host__ __device__ __forceinline__ size_t gidx(int x, int ky, int ld) { return tile_idx<XT_G>(x, ky, ld); }
+// ... of a field the row pass writes and the column pass reads
+__host__ __device__ __forceinline__ size_t nidx(int x, int ky, int ld) { return tile_idx<XT_N>(x, ky, ld); }
 
 // RK4 Butcher tableau and AB4 weights (reference Solver/src/solver.c:29-47)
 constexpr double RK4_A21 = 0.5, RK4_A32 = 0.5, RK4_A43 = 1.0;

@@ -201,8 +201,8 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int x = l + TL * e;
-      if (pair) R[e] = make_double2(a.nin[midx(x, 0, ld)].x, a.nin[midx(x, H, ld)].x);
-      else R[e] = colv ? a.nin[midx(x, ky, ld)] : make_double2(0.0, 0.0);
+      if (pair) R[e] = make_double2(a.nin[nidx(x, 0, ld)].x, a.nin[nidx(x, H, ld)].x);
+      else R[e] = colv ? a.nin[nidx(x, ky, ld)] : make_double2(0.0, 0.0);
       if ((e & 3) == 3) asm volatile("" ::: "memory");
     }
     fft_reg<L, TL, -1, BLOCK, PS>(R, buf, tw, opaque(l));
@@ -274,12 +274,16 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
       }
     }
     const bool conj_fix = (a.mode & KP_CONJFIX) && kq == 0;
+    // one load from a selected address (no branch around it, so that the
+    // loads of a thread's rows are all in flight before the first use)
     auto state_at = [&](int it) -> double2 {
       if constexpr (PF != 0) return wpre[it];
       const int i = lq + TL * it;
       const int kx = kx_of_row(i, nx);
-      if (conj_fix && kx < 0) return cconj(a.conj_src[-kx]);
-      return a.w_in[kidx(i, kq, ld, nx)];
+      const bool cj = conj_fix && kx < 0;
+      const double2* p = cj ? a.conj_src + (cj ? -kx : 0) : a.w_in + kidx(i, kq, ld, nx);
+      const double2 v = *p;
+      return make_double2(v.x, cj ? -v.y : v.y);
     };
     auto acc_at = [&](int it) -> double2 {
       if constexpr (PF != 0) return apre[it];
@@ -346,12 +350,12 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
 #pragma unroll
           for (int it = 0; it < E; ++it) {
             const int kx = kx_of_row(lq + TL * it, nx);
-            double2 nw = make_double2(0.0, 0.0);               // (0,0) mode: solver.c:860
-            if (kx != 0 || kq != 0) {
-              const double2 sum = cadd(acc_at(it), cscale(RK4_B4, R[it]));
-              const double D = a.dt * lam_at(it);
-              nw = cadd(cscale((2.0 - D) / (2.0 + D), state_at(it)), cscale(2.0 * a.dt / (2.0 + D), sum));  // :853
-            }
+            // every mode computes; the (0,0) mode then takes 0 (solver.c:860)
+            const double2 sum = cadd(acc_at(it), cscale(RK4_B4, R[it]));
+            const double D = a.dt * lam_at(it);
+            const double q = rcp_nr(2.0 + D);
+            double2 nw = cadd(cscale((2.0 - D) * q, state_at(it)), cscale(2.0 * a.dt * q, sum));  // :853
+            if (kx == 0 && kq == 0) nw = make_double2(0.0, 0.0);
             if (writer) a.w_out[index_of(it)] = nw;
             R[it] = nw;
             if ((it & 3) == 3) asm volatile("" ::: "memory");
@@ -365,13 +369,13 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
             const int kx = kx_of_row(lq + TL * it, nx);
             const size_t idx = index_of(it);
             const double2 k = R[it];
-            double2 nw = make_double2(0.0, 0.0);               // (0,0) mode: solver.c:337
-            if (kx != 0 || kq != 0) {
-              const double2 sum = cadd(cadd(cadd(cscale(AB4_1, k), cscale(AB4_2, a.h_new[idx])),
-                                            cscale(AB4_3, a.h_mid[idx])), cscale(AB4_4, a.h_old[idx]));
-              const double D = a.dt * a.lam[idx];
-              nw = cadd(cscale((2.0 - D) / (2.0 + D), state_at(it)), cscale(2.0 * a.dt / (2.0 + D), sum));  // :333
-            }
+            // every mode computes; the (0,0) mode then takes 0 (solver.c:337)
+            const double2 sum = cadd(cadd(cadd(cscale(AB4_1, k), cscale(AB4_2, a.h_new[idx])),
+                                          cscale(AB4_3, a.h_mid[idx])), cscale(AB4_4, a.h_old[idx]));
+            const double D = a.dt * a.lam[idx];
+            const double q = rcp_nr(2.0 + D);
+            double2 nw = cadd(cscale((2.0 - D) * q, state_at(it)), cscale(2.0 * a.dt * q, sum));  // :333
+            if (kx == 0 && kq == 0) nw = make_double2(0.0, 0.0);
             if (writer) {
               a.h_store[idx] = k;   // newest history entry (solver.c:346-348)
               a.w_out[idx] = nw;
@@ -415,14 +419,14 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
   if (!(a.mode & KP_INV)) return;
 
   // ---------------- INV: x-inverse FFTs of the derived fields ---------------
-  // f = 0: psi_hat = S/k^2, 1: -i kx psi_hat, 2: i kx S, 3: S
+  // f = 0: psi_hat = S/k^2, 1: -i kx psi_hat, 2: i kx S, 3: S (f is uniform:
+  // only the first two fields compute 1/k^2)
   auto derive = [](int f, double2 s, int kx, int kyv) -> double2 {
+    if (f >= 2) return f == 2 ? make_double2(-kx * s.y, kx * s.x) : s;
     const int k2i = kx * kx + kyv * kyv;
-    const double inv = k2i ? 1.0 / (double)k2i : 0.0;
+    const double inv = k2i ? rcp_nr((double)k2i) : 0.0;
     if (f == 0) return cscale(inv, s);
-    if (f == 1) return make_double2(kx * inv * s.y, -(kx * inv) * s.x);
-    if (f == 2) return make_double2(-kx * s.y, kx * s.x);
-    return s;
+    return make_double2(kx * inv * s.y, -(kx * inv) * s.x);
   };
 #pragma unroll 1
   for (int fi = 0; fi < FPS; ++fi) {
@@ -472,7 +476,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
       double* __restrict__ g4 = reinterpret_cast<double*>(a.g[3]);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const size_t r0 = midx(l_o + TL * e, 0, ld), rh = midx(l_o + TL * e, H, ld);
+        const size_t r0 = gidx(l_o + TL * e, 0, ld), rh = gidx(l_o + TL * e, H, ld);
         if (f == 0) {
           gout[r0] = make_double2(0.0, 0.0);
           gout[rh] = make_double2(0.0, F[e].x);
@@ -487,7 +491,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
       }
     } else if (colv) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) gout[midx(l_o + TL * e, ky_o, ld)] = F[e];
+      for (int e = 0; e < E; ++e) gout[gidx(l_o + TL * e, ky_o, ld)] = F[e];
     }
   }
 }
@@ -561,18 +565,18 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
       double2 A, B;
       const double fk = (double)kk;
       if (which == 0) {         // u = i ky G1, v = G2
-        const double2 g1 = a.g[0][midx(x, kk, ld)];
+        const double2 g1 = a.g[0][gidx(x, kk, ld)];
         A = make_double2(-fk * g1.y, fk * g1.x);
-        B = a.g[1][midx(x, kk, ld)];
+        B = a.g[1][gidx(x, kk, ld)];
       } else if (which == 1) {  // w_x = G3, w_y = i ky G4
-        A = a.g[2][midx(x, kk, ld)];
-        const double2 g4 = a.g[3][midx(x, kk, ld)];
+        A = a.g[2][gidx(x, kk, ld)];
+        const double2 g4 = a.g[3][gidx(x, kk, ld)];
         B = make_double2(-fk * g4.y, fk * g4.x);
       } else if (which == 2) {  // w of rows x and x+1 = G4
-        A = a.g[3][midx(x, kk, ld)];
-        B = a.g[3][midx(x + 1, kk, ld)];
+        A = a.g[3][gidx(x, kk, ld)];
+        B = a.g[3][gidx(x + 1, kk, ld)];
       } else {                  // w of row x alone
-        A = a.g[3][midx(x, kk, ld)];
+        A = a.g[3][gidx(x, kk, ld)];
         B = make_double2(0.0, 0.0);
       }
       if (kk == 0 || kk == H) { A.y = 0.0; B.y = 0.0; }
@@ -594,7 +598,7 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
           for (int e = 0; e < E; ++e) {
             const int ky = l + TLR * e;
             const int kk = ky > H ? NY - ky : ky;
-            pre[r][f][e] = a.g[f][midx(x0 + r, kk, ld)];
+            pre[r][f][e] = a.g[f][gidx(x0 + r, kk, ld)];
           }
       // the same assembly as build() from the loaded values
       auto build_pre = [&](double2* z, int which, const double2 (&g)[4][E]) {
@@ -701,7 +705,7 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = l + TLR * e;
-      if (k <= H) a.nout[midx(x0, k, ld)] = (k == 0 || k == H) ? make_double2(w[e].x, 0.0) : w[e];
+      if (k <= H) a.nout[nidx(x0, k, ld)] = (k == 0 || k == H) ? make_double2(w[e].x, 0.0) : w[e];
     }
   } else if ((a.mode & RP_NL) && (BLOCK || valid)) {
     fft_reg<NY, TLR, -1, BLOCK>(w, buf, tw, opaque(l));
@@ -715,8 +719,8 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
       if (k <= H) {
         const double2 p = w[e];
         const double2 qm = cconj(buf[padx((NY - k) & (NY - 1))]);
-        a.nout[midx(x0, k, ld)] = make_double2(0.5 * (p.x + qm.x), 0.5 * (p.y + qm.y));
-        a.nout[midx(x0 + 1, k, ld)] = make_double2(0.5 * (p.y - qm.y), -0.5 * (p.x - qm.x));
+        a.nout[nidx(x0, k, ld)] = make_double2(0.5 * (p.x + qm.x), 0.5 * (p.y + qm.y));
+        a.nout[nidx(x0 + 1, k, ld)] = make_double2(0.5 * (p.y - qm.y), -0.5 * (p.x - qm.x));
       }
     }
   }
@@ -786,7 +790,7 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
       const bool mirror = ky > H;
       const int kk = mirror ? NY - ky : ky;
       const double fk = (double)kk;
-      double2 A = fa[midx(xa, kk, ld)], B = hasb ? fb[midx(xb, kk, ld)] : make_double2(0.0, 0.0);
+      double2 A = fa[gidx(xa, kk, ld)], B = hasb ? fb[gidx(xb, kk, ld)] : make_double2(0.0, 0.0);
       if (mula) A = make_double2(-fk * A.y, fk * A.x);   // times i ky
       if (mulb) B = make_double2(-fk * B.y, fk * B.x);
       if (kk == 0 || kk == H) { A.y = 0.0; B.y = 0.0; }
@@ -813,8 +817,8 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
         const double fk = (double)k;
         if (mulp) P = make_double2(fk * P.y, -fk * P.x);   // times -i k
         if (mulq) Q = make_double2(fk * Q.y, -fk * Q.x);
-        pout[midx(x, k, ld)] = cscale(w2, P);
-        qout[midx(x, k, ld)] = cscale(w2, Q);
+        pout[nidx(x, k, ld)] = cscale(w2, P);
+        qout[nidx(x, k, ld)] = cscale(w2, Q);
       }
     }
     xsync<BLOCK>();
@@ -883,7 +887,7 @@ __global__ __launch_bounds__(T) void kpass_adj_kernel(KAdjArgs a) {
       double2 F[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        F[e] = colv ? a.bin[f][midx(l_o + TL * e, ky_o, ld)] : make_double2(0.0, 0.0);
+        F[e] = colv ? a.bin[f][nidx(l_o + TL * e, ky_o, ld)] : make_double2(0.0, 0.0);
         if ((e & 3) == 3) asm volatile("" ::: "memory");
       }
       fft_reg<L, TL, -1, BLOCK>(F, buf, tw, l_o);
@@ -891,7 +895,7 @@ __global__ __launch_bounds__(T) void kpass_adj_kernel(KAdjArgs a) {
       for (int e = 0; e < E; ++e) {
         const int kx = kx_of_row(l_o + TL * e, nx);
         const int k2i = kx * kx + ky_o * ky_o;
-        const double inv = k2i ? 1.0 / (double)k2i : 0.0;
+        const double inv = k2i ? rcp_nr((double)k2i) : 0.0;
         const double2 v = F[e];
         double2 t;
         if (f == 0) t = cscale(inv, v);                                   // conj(U): 1/|k|^2 (-i ky in B1)
@@ -913,24 +917,26 @@ __global__ __launch_bounds__(T) void kpass_adj_kernel(KAdjArgs a) {
     const size_t idx = kidx(l + TL * e, ky, ld, nx);
     const bool zero = (kx == 0 && ky == 0);
     const double D = h * a.lam[idx];
-    const double A = (2.0 - D) / (2.0 + D), B = 2.0 * h / (2.0 + D);
-    double2 lp = zero ? make_double2(0.0, 0.0) : a.lamp_in[idx];   // dJ/dw_{n+1}
+    const double q = rcp_nr(2.0 + D);
+    const double A = (2.0 - D) * q, B = 2.0 * h * q;
+    double2 lp = a.lamp_in[idx];   // dJ/dw_{n+1}
+    if (zero) lp = make_double2(0.0, 0.0);
     switch (a.epi) {
       case ADJ_START:
         a.lacc[idx] = cscale(A, lp);
-        LK[e] = cscale(B / 6.0, lp);
+        LK[e] = cscale(B * RK4_B1, lp);
         break;
       case ADJ_S4:
         a.lacc[idx] = cadd(a.lacc[idx], LSacc[e]);
-        LK[e] = cadd(cscale(B / 3.0, lp), cscale(h, LSacc[e]));
+        LK[e] = cadd(cscale(B * RK4_B2, lp), cscale(h, LSacc[e]));
         break;
       case ADJ_S3:
         a.lacc[idx] = cadd(a.lacc[idx], LSacc[e]);
-        LK[e] = cadd(cscale(B / 3.0, lp), cscale(0.5 * h, LSacc[e]));
+        LK[e] = cadd(cscale(B * RK4_B2, lp), cscale(0.5 * h, LSacc[e]));
         break;
       case ADJ_S2:
         a.lacc[idx] = cadd(a.lacc[idx], LSacc[e]);
-        LK[e] = cadd(cscale(B / 6.0, lp), cscale(0.5 * h, LSacc[e]));
+        LK[e] = cadd(cscale(B * RK4_B1, lp), cscale(0.5 * h, LSacc[e]));
         break;
       case ADJ_S1: {
         // dJ/dw_n; with KA_INV it starts the adjoint of the previous step
@@ -938,10 +944,11 @@ __global__ __launch_bounds__(T) void kpass_adj_kernel(KAdjArgs a) {
         a.lamp_out[idx] = lw;
         if (a.mode & KA_INV) {
           const double Dp = h * a.lam_prev[idx];
-          const double Ap = (2.0 - Dp) / (2.0 + Dp), Bp = 2.0 * h / (2.0 + Dp);
+          const double qp = rcp_nr(2.0 + Dp);
+          const double Ap = (2.0 - Dp) * qp, Bp = 2.0 * h * qp;
           const double2 lq = zero ? make_double2(0.0, 0.0) : lw;
           a.lacc[idx] = cscale(Ap, lq);
-          LK[e] = cscale(Bp / 6.0, lq);
+          LK[e] = cscale(Bp * RK4_B1, lq);
         }
         break;
       }
@@ -965,7 +972,7 @@ __global__ __launch_bounds__(T) void kpass_adj_kernel(KAdjArgs a) {
     fft_reg<L, TL, +1, BLOCK>(LK, buf, tw, l_o);
     if (colv) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) a.lout[midx(l_o + TL * e, ky_o, ld)] = LK[e];
+      for (int e = 0; e < E; ++e) a.lout[gidx(l_o + TL * e, ky_o, ld)] = LK[e];
     }
   }
 }
@@ -1005,7 +1012,7 @@ __global__ __launch_bounds__(T) void r2c_rows_kernel(const double* __restrict__ 
         const double2 O = make_double2(0.5 * (p.y - q.y), -0.5 * (p.x - q.x));
         X = cadd(E, cmul(tw[k * TWS], O));
       }
-      out[midx(x, k, ld)] = X;
+      out[nidx(x, k, ld)] = X;
     }
   }
 }

@@ -123,7 +123,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ns2d_variant_extra": ([i, i, i, P(i)], i),
         "ns2d_default_variant": ([i, i, i, P(i)], i),
         "ns2d_set_variant": ([vp, i, i], i),
-        "ns2d_layout": ([P(i), P(i)], i),
+        "ns2d_layout": ([P(i), P(i), P(i)], i),
         "ns2d_ens_create": ([P(_Params), i, P(vp)], i),
         "ns2d_ens_member": ([vp, i, P(vp)], i),
         "ns2d_ens_step": ([vp, dp, l], i),
@@ -184,12 +184,13 @@ def default_variant(pass_kind: int, n: int, integrator: str = "RK4CN") -> int:
 
 
 def layout() -> tuple:
-    """(k-space x-tile, mixed-space x-tile) the library stores its device
-    arrays in; k-space 0 = column-major [Ny/2+1][Nx]."""
+    """(k-space x-tile, column-to-row mixed-space x-tile, row-to-column
+    mixed-space x-tile) the library stores its device arrays in; k-space 0 =
+    column-major [Ny/2+1][Nx]."""
     lib = load_library()
-    k, m = ctypes.c_int(), ctypes.c_int()
-    _check(lib.ns2d_layout(ctypes.byref(k), ctypes.byref(m)))
-    return k.value, m.value
+    k, g, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(lib.ns2d_layout(ctypes.byref(k), ctypes.byref(g), ctypes.byref(n)))
+    return k.value, g.value, n.value
 
 
 class NS2DError(RuntimeError):

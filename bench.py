@@ -69,10 +69,11 @@ def kernel_signature(pass_name, n):
 
 def traffic_key(pass_name, n):
     """kernel_signature plus the device layout the library was compiled with
-    (k-space x-tile K, 0 = column-major; mixed-space x-tile M): "<sig> layout kKmM"."""
+    (k-space x-tile K, 0 = column-major; x-tiles G and N of the mixed-space
+    fields each pass writes for the other): "<sig> layout kKgGnN"."""
     import ns2d
-    k, m = ns2d.layout()
-    return f"{kernel_signature(pass_name, n)} layout k{k}m{m}"
+    k, g, m = ns2d.layout()
+    return f"{kernel_signature(pass_name, n)} layout k{k}g{g}n{m}"
 
 
 def make_ic(nx, ny, seed=1):

@@ -196,11 +196,12 @@ int ns2d_default_variant(int pass, int n, int integrator, int* idx);
 int ns2d_set_variant(ns2d_t* h, int pass, int idx);
 /* Device storage of the spectral arrays this library was compiled with:
  * *xtile_k for the k-space arrays (state, stage fields, mode tables), 0 =
- * column-major [Ny/2+1][Nx], else rows of x-tiles of that many x; *xtile_m
- * for the mixed-space fields between the two passes (x-tile rows).  The host
+ * column-major [Ny/2+1][Nx], else rows of x-tiles of that many x; *xtile_g
+ * for the mixed-space fields the column pass writes for the row pass, *xtile_n
+ * for those the row pass writes for the column pass (x-tile rows).  The host
  * side always sees [Nx][Ny/2+1].  Needs no GPU.  No counterpart in the
  * reference (FFTW's layout is fixed by its plans). */
-int ns2d_layout(int* xtile_k, int* xtile_m);
+int ns2d_layout(int* xtile_k, int* xtile_g, int* xtile_n);
 
 /* Ensemble (no counterpart in the reference, which runs one realisation per
  * process): n independent RK4CN solvers of possibly different grid sizes,

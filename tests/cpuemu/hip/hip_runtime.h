@@ -59,6 +59,8 @@ void emu_wave_arrive(unsigned wave);
 void emu_syncthreads();
 inline void __syncthreads() { emu_syncthreads(); }
 #define __builtin_amdgcn_fence(...) ((void)0)
+// the hardware reciprocal's stand-in (the kernels refine it by Newton steps)
+inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
 inline void __builtin_amdgcn_wave_barrier() { emu_wave_arrive(threadIdx.x / 64); }
 extern double emu_shfl_d[1024];
 extern unsigned long long emu_shfl_u[1024];

@@ -306,7 +306,7 @@ def test_measures_workgroup_to_bin_map(nbins):
 
 @pytest.mark.parametrize("xt", [1, 2, 4, 8])
 def test_spectral_tile_index_is_a_permutation(xt):
-    """tile_idx (ns2d_internal.h, the kidx / midx layouts): with x-tiles of XT
+    """tile_idx (ns2d_internal.h, the kidx / gidx / nidx layouts): with x-tiles of XT
     rows, element (x, ky) of an [Nx][ld] array sits at ((x/XT)*ld + ky)*XT +
     x%XT -- a permutation of [0, Nx*ld) for every grid the ABI accepts (XT <= 8
     divides every Nx), so every spectral buffer keeps its size."""
