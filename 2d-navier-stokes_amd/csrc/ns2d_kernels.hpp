@@ -556,11 +556,16 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
 
   // full-length spectrum of A + iB for row x (which: 0 -> u + iv, 1 -> w_x + i w_y,
   // 2 -> w(x) + i w(x+1), 3 -> w(x))
+  // Element e of lane lo holds ky = lo + TLR*e: elements below E/2 lie in
+  // [0, H), those above E/2 in (H, NY), and only lane 0's elements 0 and E/2
+  // are the real-valued ky = 0 and ky = H, so the mirror and edge tests are
+  // fixed per element once the loop is unrolled.
   auto build = [&](double2* z, int which, int x, int lo) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int ky = lo + TLR * e;
-      const bool mirror = ky > H;
+      const bool edge = (e == 0 || e == E / 2) && lo == 0;
+      const bool mirror = e > E / 2 || (e == E / 2 && lo != 0);
       const int kk = mirror ? NY - ky : ky;
       double2 A, B;
       const double fk = (double)kk;
@@ -579,7 +584,7 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
         A = a.g[3][gidx(x, kk, ld)];
         B = make_double2(0.0, 0.0);
       }
-      if (kk == 0 || kk == H) { A.y = 0.0; B.y = 0.0; }
+      if (edge) { A.y = 0.0; B.y = 0.0; }
       z[e] = mirror ? make_double2(A.x + B.y, B.x - A.y) : make_double2(A.x - B.y, A.y + B.x);
       if ((e & 3) == 3) asm volatile("" ::: "memory");
     }
@@ -702,10 +707,12 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
 
   if (SR && (a.mode & RP_NL) && (BLOCK || valid)) {
     fft_reg<NY, TLR, -1, BLOCK>(w, buf, tw, opaque(l));
+    // k = l + TLR*e <= H: every element below E/2, lane 0's element E/2 (k = H)
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
+    for (int e = 0; e <= E / 2; ++e) {
       const int k = l + TLR * e;
-      if (k <= H) a.nout[nidx(x0, k, ld)] = (k == 0 || k == H) ? make_double2(w[e].x, 0.0) : w[e];
+      if (e < E / 2) a.nout[nidx(x0, k, ld)] = (e == 0 && l == 0) ? make_double2(w[e].x, 0.0) : w[e];
+      else if (l == 0) a.nout[nidx(x0, k, ld)] = make_double2(w[e].x, 0.0);
     }
   } else if ((a.mode & RP_NL) && (BLOCK || valid)) {
     fft_reg<NY, TLR, -1, BLOCK>(w, buf, tw, opaque(l));
@@ -713,10 +720,11 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
 #pragma unroll
     for (int e = 0; e < E; ++e) buf[padx(l + TLR * e)] = w[e];
     xsync<BLOCK>();
+    // k = l + TLR*e <= H: every element below E/2, lane 0's element E/2
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
+    for (int e = 0; e <= E / 2; ++e) {
       const int k = l + TLR * e;
-      if (k <= H) {
+      if (e < E / 2 || l == 0) {
         const double2 p = w[e];
         const double2 qm = cconj(buf[padx((NY - k) & (NY - 1))]);
         a.nout[nidx(x0, k, ld)] = make_double2(0.5 * (p.x + qm.x), 0.5 * (p.y + qm.y));
@@ -782,18 +790,21 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
   // full-length spectrum of A + iB from the half spectra of two real fields,
   // A from row xa of fa, B from row xb of fb (fb == nullptr only where hasb
   // is 0: B = 0)
+  // (element e of lane lo holds ky = lo + TLR*e; the mirror and edge tests
+  // are fixed per element as in the forward row pass)
   auto build = [&](double2* z, const double2* fa, int xa, const double2* fb, int xb, int mula, int mulb, int lo,
                    int hasb = 1) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int ky = lo + TLR * e;
-      const bool mirror = ky > H;
+      const bool edge = (e == 0 || e == E / 2) && lo == 0;
+      const bool mirror = e > E / 2 || (e == E / 2 && lo != 0);
       const int kk = mirror ? NY - ky : ky;
       const double fk = (double)kk;
       double2 A = fa[gidx(xa, kk, ld)], B = hasb ? fb[gidx(xb, kk, ld)] : make_double2(0.0, 0.0);
       if (mula) A = make_double2(-fk * A.y, fk * A.x);   // times i ky
       if (mulb) B = make_double2(-fk * B.y, fk * B.x);
-      if (kk == 0 || kk == H) { A.y = 0.0; B.y = 0.0; }
+      if (edge) { A.y = 0.0; B.y = 0.0; }
       z[e] = mirror ? make_double2(A.x + B.y, B.x - A.y) : make_double2(A.x - B.y, A.y + B.x);
       if ((e & 3) == 3) asm volatile("" ::: "memory");
     }
@@ -1050,42 +1061,58 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
       long yhi = isqrt_floor(hi - kx2);
       if (yhi > H) yhi = H;
       const int i = kx >= 0 ? kx : kx + nx;
-      for (long ky = ylo; ky <= yhi; ++ky) {
-        const size_t idx = kidx(i, (int)ky, a.ld, nx);
-        const double2 w = a.w_hat[idx];
-        const double k2 = (double)(kx2 + ky * ky);
-        const double c = (ky == 0 || ky == H) ? 1.0 : 2.0;
-        const double a2 = w.x * w.x + w.y * w.y;
-        const double pre = a.lam[idx];
-        acc[Q_ENRG] += c * a2 / k2;
-        acc[Q_ENST] += c * a2;
-        acc[Q_PALIN] += c * k2 * a2;
-        // |u|^2+|v|^2 with u = i ky w/k^2, v = -i kx w/k^2
-        const double uv2 = (double)(ky * ky) * a2 / (k2 * k2) + (double)kx2 * a2 / (k2 * k2);
-        acc[Q_ENRG_DISS] += c * pre * uv2;
-        acc[Q_ENST_DISS] += c * ((a.quirk_enst_diss && c == 2.0) ? 1.0 : pre) * a2;
-        {
-          // |i (kx u + ky v)|^2, sys_msr.c:186-193 (zero up to rounding)
-          const double2 u = make_double2(-(double)ky * w.y / k2, (double)ky * w.x / k2);
-          const double2 v = make_double2((double)kx * w.y / k2, -(double)kx * w.x / k2);
-          const double dx = (double)kx * u.x + (double)ky * v.x, dy = (double)kx * u.y + (double)ky * v.y;
-          acc[Q_DIV] += c * (dx * dx + dy * dy);
+      // four ky per round: the loads of a round are issued together, then the
+      // modes are summed in ky order (the order of a one-by-one walk)
+      for (long ky0 = ylo; ky0 <= yhi; ky0 += 4) {
+        double2 wv[4], rv[4];
+        double pv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const long kyl = ky0 + u <= yhi ? ky0 + u : yhi;   // a valid index
+          const size_t idx = kidx(i, (int)kyl, a.ld, nx);
+          wv[u] = a.w_hat[idx];
+          pv[u] = a.lam[idx];
+          rv[u] = a.rhs ? a.rhs[idx] : make_double2(0.0, 0.0);
         }
-        if (a.rhs) {
-          double2 r = a.rhs[idx];
-          // the flux terms ignore the forcing (sys_msr.c:128-132)
-          if (a.ph.force_on && kx == 0 && ky == a.ph.force_k) r.x -= a.ph.force_val;
-          const double deriv = 2.0 * (w.x * r.x + w.y * r.y);
-          const double diss = pre * a2;
-          acc[Q_DERIV] += c * deriv;
-          acc[Q_DISS] += c * diss;
-          acc[Q_DERIV_K2] += c * deriv / k2;
-          acc[Q_DISS_K2] += c * diss / k2;
-          if (k2 < a.sbst_hi && k2 >= a.sbst_lo) {
-            acc[Q_SB_DERIV] += c * deriv;
-            acc[Q_SB_DISS] += c * diss;
-            acc[Q_SB_DERIV_K2] += c * deriv / k2;
-            acc[Q_SB_DISS_K2] += c * diss / k2;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const long ky = ky0 + u;
+          if (ky > yhi) break;
+          const double2 w = wv[u];
+          const double k2 = (double)(kx2 + ky * ky);
+          const double c = (ky == 0 || ky == H) ? 1.0 : 2.0;
+          const double a2 = w.x * w.x + w.y * w.y;
+          const double pre = pv[u];
+          acc[Q_ENRG] += c * a2 / k2;
+          acc[Q_ENST] += c * a2;
+          acc[Q_PALIN] += c * k2 * a2;
+          // |u|^2+|v|^2 with u = i ky w/k^2, v = -i kx w/k^2
+          const double uv2 = (double)(ky * ky) * a2 / (k2 * k2) + (double)kx2 * a2 / (k2 * k2);
+          acc[Q_ENRG_DISS] += c * pre * uv2;
+          acc[Q_ENST_DISS] += c * ((a.quirk_enst_diss && c == 2.0) ? 1.0 : pre) * a2;
+          {
+            // |i (kx u + ky v)|^2, sys_msr.c:186-193 (zero up to rounding)
+            const double2 uu = make_double2(-(double)ky * w.y / k2, (double)ky * w.x / k2);
+            const double2 vv = make_double2((double)kx * w.y / k2, -(double)kx * w.x / k2);
+            const double dx = (double)kx * uu.x + (double)ky * vv.x, dy = (double)kx * uu.y + (double)ky * vv.y;
+            acc[Q_DIV] += c * (dx * dx + dy * dy);
+          }
+          if (a.rhs) {
+            double2 r = rv[u];
+            // the flux terms ignore the forcing (sys_msr.c:128-132)
+            if (a.ph.force_on && kx == 0 && ky == a.ph.force_k) r.x -= a.ph.force_val;
+            const double deriv = 2.0 * (w.x * r.x + w.y * r.y);
+            const double diss = pre * a2;
+            acc[Q_DERIV] += c * deriv;
+            acc[Q_DISS] += c * diss;
+            acc[Q_DERIV_K2] += c * deriv / k2;
+            acc[Q_DISS_K2] += c * diss / k2;
+            if (k2 < a.sbst_hi && k2 >= a.sbst_lo) {
+              acc[Q_SB_DERIV] += c * deriv;
+              acc[Q_SB_DISS] += c * diss;
+              acc[Q_SB_DERIV_K2] += c * deriv / k2;
+              acc[Q_SB_DISS_K2] += c * diss / k2;
+            }
           }
         }
       }
