@@ -83,21 +83,13 @@ hipError_t launch_kpass_batched(int nx, int nyf, int variant, const KPassArgs* t
   return hipGetLastError();
 }
 
-// Default column-pass shape: index 0 of its length, except CN-AB4 at 4096.
-// Its epilogue reads three history fields besides the state, and with them
-// 512 threads of 16 rows each on unpacked columns beat the packed 1024-thread
-// shape (464 vs 475 us per launch on MI355X), which the lighter RK4CN stage
-// epilogues prefer (342 vs 344 us).
+// Default column-pass shape: index 0 of its length for both integrators (at
+// 4096 the unpacked 512-thread shape with the 16-slot padding wins for the
+// RK4CN stages, 327 against 332 us per launch for the packed 1024-thread one,
+// and for CN-AB4's heavier epilogue, 455 against 457 us).
 int kpass_default_variant(int nx, bool ab4) {
-  if (ab4 && nx == 4096) {
-    int n = 0;
-    for (const KVariant& v : kvariant_table()) {
-      if (v.L != nx) continue;
-      if (v.C == 2 && v.T == 512 && v.KS == 1 && v.PS == 3 && v.OFF == 1 && v.PK == 0 && v.W == 1 && v.PF == 0)
-        return n;
-      ++n;
-    }
-  }
+  (void)nx;
+  (void)ab4;
   return 0;
 }
 

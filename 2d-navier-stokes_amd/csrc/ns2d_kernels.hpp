@@ -512,6 +512,12 @@ __global__ __launch_bounds__(T, W) void kpass_batched(const KPassArgs* __restric
   kpass_body<L, C, T, KS, PS, OFF, PK, PF>(tab[m], b - m * per_member);
 }
 
+// LDS padding of the row passes' line buffers: one 16-B slot per 2^RPS
+#ifndef NS2D_RPASS_PS
+#define NS2D_RPASS_PS 3
+#endif
+constexpr int RPS = NS2D_RPASS_PS;
+
 // ---------------------------------------------------------------------------
 // rpass: row pass in (x, ky) space, one row pair (x0, x0+1) per TLR threads
 //
@@ -538,7 +544,7 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   constexpr int T = TLR * PAIRS;
   constexpr int H = NY / 2;
   constexpr int E = NY / TLR;
-  constexpr int LS = line_stride(NY);
+  constexpr int LS = padx_s<RPS>(NY) + 1;
   constexpr bool BLOCK = TLR > 64;
   constexpr bool SR = PFR >= 2;
   static_assert(E >= 2 && (BLOCK ? PAIRS == 1 : 64 % TLR == 0), "bad rpass shape");
@@ -629,9 +635,9 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
       for (int r = 0; r < 2; ++r) {
         double2 z1[E], z2[E];
         build_pre(z1, 0, pre[r]);
-        fft_reg<NY, TLR, +1, BLOCK>(z1, buf, tw, opaque(l));
+        fft_reg<NY, TLR, +1, BLOCK, RPS>(z1, buf, tw, opaque(l));
         build_pre(z2, 1, pre[r]);
-        fft_reg<NY, TLR, +1, BLOCK>(z2, buf, tw, opaque(l));
+        fft_reg<NY, TLR, +1, BLOCK, RPS>(z2, buf, tw, opaque(l));
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           const double nl = z1[e].x * z2[e].x + z1[e].y * z2[e].y;   // u*w_x + v*w_y
@@ -644,9 +650,9 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
     double2 z1[E], z2[E];
     build(z1, 0, x0, opaque(l));
     if constexpr (PFR == 3) build(z2, 1, x0, opaque(l));   // both field pairs loaded up front
-    fft_reg<NY, TLR, +1, BLOCK>(z1, buf, tw, opaque(l));
+    fft_reg<NY, TLR, +1, BLOCK, RPS>(z1, buf, tw, opaque(l));
     if constexpr (PFR != 3) build(z2, 1, x0, opaque(l));
-    fft_reg<NY, TLR, +1, BLOCK>(z2, buf, tw, opaque(l));
+    fft_reg<NY, TLR, +1, BLOCK, RPS>(z2, buf, tw, opaque(l));
 #pragma unroll
     for (int e = 0; e < E; ++e) w[e] = make_double2(z1[e].x * z2[e].x + z1[e].y * z2[e].y, 0.0);
   } else if ((a.mode & RP_NL) && (BLOCK || valid)) {
@@ -655,9 +661,9 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
       const int x = opaque(x0 + r);
       double2 z1[E], z2[E];
       build(z1, 0, x, opaque(l));
-      fft_reg<NY, TLR, +1, BLOCK>(z1, buf, tw, opaque(l));
+      fft_reg<NY, TLR, +1, BLOCK, RPS>(z1, buf, tw, opaque(l));
       build(z2, 1, x, opaque(l));
-      fft_reg<NY, TLR, +1, BLOCK>(z2, buf, tw, opaque(l));
+      fft_reg<NY, TLR, +1, BLOCK, RPS>(z2, buf, tw, opaque(l));
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const double nl = z1[e].x * z2[e].x + z1[e].y * z2[e].y;   // u*w_x + v*w_y
@@ -672,7 +678,7 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
     if (BLOCK || valid) {
       double2 z[E];
       build(z, SR ? 3 : 2, x0, opaque(l));
-      fft_reg<NY, TLR, +1, BLOCK>(z, buf, tw, opaque(l));
+      fft_reg<NY, TLR, +1, BLOCK, RPS>(z, buf, tw, opaque(l));
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int y = l + TLR * e;
@@ -706,7 +712,7 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   }
 
   if (SR && (a.mode & RP_NL) && (BLOCK || valid)) {
-    fft_reg<NY, TLR, -1, BLOCK>(w, buf, tw, opaque(l));
+    fft_reg<NY, TLR, -1, BLOCK, RPS>(w, buf, tw, opaque(l));
     // k = l + TLR*e <= H: every element below E/2, lane 0's element E/2 (k = H)
 #pragma unroll
     for (int e = 0; e <= E / 2; ++e) {
@@ -715,10 +721,10 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
       else if (l == 0) a.nout[nidx(x0, k, ld)] = make_double2(w[e].x, 0.0);
     }
   } else if ((a.mode & RP_NL) && (BLOCK || valid)) {
-    fft_reg<NY, TLR, -1, BLOCK>(w, buf, tw, opaque(l));
+    fft_reg<NY, TLR, -1, BLOCK, RPS>(w, buf, tw, opaque(l));
     // W[NY-k] for the thread's own k: exchange through the line buffer
 #pragma unroll
-    for (int e = 0; e < E; ++e) buf[padx(l + TLR * e)] = w[e];
+    for (int e = 0; e < E; ++e) buf[padx_s<RPS>(l + TLR * e)] = w[e];
     xsync<BLOCK>();
     // k = l + TLR*e <= H: every element below E/2, lane 0's element E/2
 #pragma unroll
@@ -726,7 +732,7 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
       const int k = l + TLR * e;
       if (e < E / 2 || l == 0) {
         const double2 p = w[e];
-        const double2 qm = cconj(buf[padx((NY - k) & (NY - 1))]);
+        const double2 qm = cconj(buf[padx_s<RPS>((NY - k) & (NY - 1))]);
         a.nout[nidx(x0, k, ld)] = make_double2(0.5 * (p.x + qm.x), 0.5 * (p.y + qm.y));
         a.nout[nidx(x0 + 1, k, ld)] = make_double2(0.5 * (p.y - qm.y), -0.5 * (p.x - qm.x));
       }
@@ -772,7 +778,7 @@ template <int NY, int TLR, int PAIRS, int SR>
 __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk) {
   constexpr int H = NY / 2;
   constexpr int E = NY / TLR;
-  constexpr int LS = line_stride(NY);
+  constexpr int LS = padx_s<RPS>(NY) + 1;
   constexpr bool BLOCK = TLR > 64;
   static_assert(E >= 2 && (BLOCK ? PAIRS == 1 : 64 % TLR == 0), "bad rpass_adj shape");
   static_assert(!SR || PAIRS == 1, "single-row shapes own one row per workgroup");
@@ -814,14 +820,14 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
   // W2 (mul ? -i k : 1) P to pout and W2 (mulq ? -i k : 1) Q to qout
   auto split_store = [&](double2* z, double2* pout, double2* qout, int x, int mulp, int mulq, int lo) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) buf[padx(lo + TLR * e)] = z[e];
+    for (int e = 0; e < E; ++e) buf[padx_s<RPS>(lo + TLR * e)] = z[e];
     xsync<BLOCK>();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = lo + TLR * e;
       if (k <= H) {
         const double2 zp = z[e];
-        const double2 zm = cconj(buf[padx((NY - k) & (NY - 1))]);
+        const double2 zm = cconj(buf[padx_s<RPS>((NY - k) & (NY - 1))]);
         double2 P = make_double2(0.5 * (zp.x + zm.x), 0.5 * (zp.y + zm.y));
         double2 Q = make_double2(0.5 * (zp.y - zm.y), -0.5 * (zp.x - zm.x));
         const double w2 = (k == 0 || k == H) ? 1.0 : 2.0;
@@ -839,15 +845,15 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
   double2 zl[E];
   if constexpr (SR) build(zl, a.lin, x0, nullptr, x0, 0, 0, opaque(l), 0);
   else build(zl, a.lin, x0, a.lin, x0 + 1, 0, 0, opaque(l));
-  fft_reg<NY, TLR, +1, BLOCK>(zl, buf, tw, opaque(l));
+  fft_reg<NY, TLR, +1, BLOCK, RPS>(zl, buf, tw, opaque(l));
 #pragma unroll 1
   for (int r = 0; r < (SR ? 1 : 2); ++r) {
     const int x = opaque(x0 + r);
     double2 z1[E], z2[E];
     build(z1, a.g[0], x, a.g[1], x, 1, 0, opaque(l));   // u + i v
-    fft_reg<NY, TLR, +1, BLOCK>(z1, buf, tw, opaque(l));
+    fft_reg<NY, TLR, +1, BLOCK, RPS>(z1, buf, tw, opaque(l));
     build(z2, a.g[2], x, a.g[3], x, 0, 1, opaque(l));   // w_x + i w_y
-    fft_reg<NY, TLR, +1, BLOCK>(z2, buf, tw, opaque(l));
+    fft_reg<NY, TLR, +1, BLOCK, RPS>(z2, buf, tw, opaque(l));
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const double ln = (r == 0) ? zl[e].x : zl[e].y;
@@ -855,9 +861,9 @@ __device__ __forceinline__ void rpass_adj_body(const RAdjArgs& a, const int blk)
       z1[e] = make_double2(ln * wv.x, ln * u.x);   // l_n w_x + i l_n u
       z2[e] = make_double2(ln * wv.y, ln * u.y);   // l_n w_y + i l_n v
     }
-    fft_reg<NY, TLR, -1, BLOCK>(z1, buf, tw, opaque(l));
+    fft_reg<NY, TLR, -1, BLOCK, RPS>(z1, buf, tw, opaque(l));
     split_store(z1, a.bout[0], a.bout[1], x, 1, 0, opaque(l));
-    fft_reg<NY, TLR, -1, BLOCK>(z2, buf, tw, opaque(l));
+    fft_reg<NY, TLR, -1, BLOCK, RPS>(z2, buf, tw, opaque(l));
     split_store(z2, a.bout[2], a.bout[3], x, 0, 1, opaque(l));
   }
 }
@@ -1141,16 +1147,18 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
 // into slot 0 (PK = 2: with the pair's second column in registers), W minimum
 // waves per SIMD, PF = 1 issues the RK epilogue's loads before the forward
 // transform; the first listed for each L is the default.  Rows per thread
-// E = L*C/T.  Measured on square grids: up to L = 2048 the unpacked shape is
-// faster (the packed pair's extra exchanges and held columns make slot 0 the
-// last workgroup to finish, and at 2048 its register budget costs
-// occupancy); at 4096, one workgroup per CU, the packed pair removes the odd
-// column group of the last round (-9 %), and 1024 threads with 8 rows each
-// and a 4-slot line offset beat 512 threads with 16 rows (-3 %; CN-AB4 keeps
-// the latter, kpass_default_variant).  At 1024, 512 threads with 4 rows each
-// beat 256 threads with 8 rows (-2 %, shorter per-workgroup chains); at 512,
-// two columns per 256 threads beat four (-12 %), at 256 eight columns per 512
-// threads beat 256 threads (-8 %).
+// E = L*C/T.  Measured on square grids: the unpacked shape is the default at
+// every length (the packed pair's extra exchanges and held columns make slot
+// 0 the last workgroup to finish; at 4096, one workgroup per CU, it removes
+// the odd column group of the last round, which paid off with the 8-slot
+// padding but no longer with the 16-slot one).  Padding one slot per 16
+// (PS = 4) with the two columns' lines 8 slots apart (OFF = 8: adjacent lanes
+// work on the two columns, whose accesses then fall on opposite halves of
+// the banks) beats one slot per 8 with adjacent lines by 6 % at 1024 and 2048
+// and 4 % at 4096.  At 1024, 512 threads with 4 rows each beat 256 threads
+// with 8 rows (shorter per-workgroup chains); at 512, two columns per 256
+// threads beat four (-12 %), at 256 eight columns per 512 threads beat 256
+// threads (-8 %).
 #define NS2D_KPASS_VARIANTS(X) \
   X(8, 16, 64, 1, 3, 1, 0, 1, 0) \
   X(8, 16, 64, 1, 3, 1, 1, 1, 0) \
@@ -1187,6 +1195,9 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(512, 16, 512, 2, 3, 1, 0, 1, 0) \
   X(512, 4, 256, 2, 3, 1, 2, 1, 0) \
   X(512, 4, 512, 2, 3, 1, 0, 1, 0) \
+  X(512, 2, 256, 2, 3, 8, 0, 1, 0) \
+  X(512, 2, 256, 2, 4, 8, 0, 1, 0) \
+  X(1024, 2, 512, 2, 4, 8, 0, 1, 0) \
   X(1024, 2, 512, 2, 3, 1, 0, 1, 0) \
   X(1024, 2, 256, 2, 3, 1, 0, 1, 0) \
   X(1024, 2, 256, 2, 3, 1, 0, 1, 1) \
@@ -1215,6 +1226,12 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(1024, 2, 256, 2, 3, 1, 2, 3, 0) \
   X(1024, 2, 256, 4, 3, 1, 0, 1, 0) \
   X(1024, 2, 512, 4, 3, 1, 0, 1, 0) \
+  X(1024, 2, 512, 2, 3, 8, 0, 1, 0) \
+  X(1024, 2, 512, 2, 2, 8, 0, 1, 0) \
+  X(1024, 2, 512, 2, 4, 1, 0, 1, 0) \
+  X(1024, 2, 256, 2, 3, 8, 0, 1, 0) \
+  X(1024, 2, 256, 2, 4, 8, 0, 1, 0) \
+  X(2048, 2, 512, 1, 4, 8, 0, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 0, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 0, 1, 1) \
   X(2048, 2, 512, 1, 3, 1, 1, 4, 0) \
@@ -1225,6 +1242,8 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(2048, 4, 1024, 1, 3, 1, 1, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 2, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 2, 4, 0) \
+  X(2048, 2, 512, 1, 3, 8, 0, 1, 0) \
+  X(4096, 2, 512, 1, 4, 8, 0, 1, 0) \
   X(4096, 2, 1024, 1, 3, 4, 1, 1, 0) \
   X(4096, 2, 512, 1, 3, 1, 1, 1, 0) \
   X(4096, 2, 512, 1, 4, 4, 1, 1, 0) \
@@ -1237,7 +1256,9 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(4096, 2, 1024, 1, 3, 8, 1, 1, 0) \
   X(4096, 1, 512, 1, 3, 1, 1, 1, 0) \
   X(4096, 1, 512, 2, 3, 1, 1, 1, 0) \
-  X(4096, 2, 512, 1, 3, 1, 0, 1, 0)
+  X(4096, 2, 512, 1, 3, 1, 0, 1, 0) \
+  X(4096, 2, 1024, 1, 4, 8, 1, 1, 0) \
+  X(4096, 2, 512, 1, 3, 8, 0, 1, 0)
 
 struct KVariant {
   int L, C, T, KS, PS, OFF, PK, W, PF;

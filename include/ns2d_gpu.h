@@ -173,8 +173,8 @@ int ns2d_profile_steps(ns2d_t* h, double dt, long nsteps, double* kpass_ms, doub
  *   row pass (Ny):    {threads per row pair, points per thread, threads,
  *                      row pairs per workgroup}
  * The project's autotuner times them and picks one per grid; a new handle
- * starts from ns2d_default_variant's shape (index 0, except the column pass
- * of CN-AB4 at Nx = 4096).  No counterpart in the reference (FFTW_MEASURE
+ * starts from ns2d_default_variant's shape (index 0 for both integrators;
+ * the integrator argument lets a table name different ones).  No counterpart in the reference (FFTW_MEASURE
  * planning, solver.c:2883-2895, plays this role there). */
 #define NS2D_PASS_COLUMN 0
 #define NS2D_PASS_ROW 1

@@ -30,23 +30,20 @@ def test_variant_tables_cover_every_size():
         for v in vs:
             assert v["threads"] % v["cols"] == 0 and 4 % v["splits"] == 0
             assert v["per_thread"] * v["threads"] == n * v["cols"]
-        # every length keeps an unpacked shape (the packed pair is an option),
-        # the prefetching epilogue only on unpacked shapes, defaults unpacked
-        # below 4096 (measured) and packed at 4096 (1024 instead of 1025 work items)
+        # every length keeps an unpacked shape (the packed pair is an option,
+        # at 4096 among others), the prefetching epilogue only on unpacked
+        # shapes, defaults unpacked (measured)
         assert any(not v["packed_pair"] for v in vs), n
         assert not any(v["packed_pair"] and v["prefetch"] for v in vs), n
-        assert vs[0]["packed_pair"] == (1 if n == 4096 else 0), n
-        assert all(v["min_waves"] >= 1 and v["pad_shift"] >= 3 for v in vs), n
-        # defaults: index 0 of each length, except CN-AB4's column pass at
-        # 4096 (an unpacked 512-thread shape; RK4CN's is the packed 1024-thread one)
-        assert ns2d.default_variant(ns2d.PASS_ROW, n, "AB4") == 0
-        assert ns2d.default_variant(ns2d.PASS_COLUMN, n, "RK4CN") == 0
-        ab4 = ns2d.default_variant(ns2d.PASS_COLUMN, n, "AB4")
+        assert vs[0]["packed_pair"] == 0, n
         if n == 4096:
-            assert ab4 > 0 and vs[ab4]["threads"] == 512 and vs[ab4]["cols"] == 2 and not vs[ab4]["packed_pair"]
-            assert vs[ab4]["splits"] == 1 and not vs[ab4]["prefetch"] and vs[0]["threads"] == 1024
-        else:
-            assert ab4 == 0, n
+            assert any(v["packed_pair"] for v in vs)
+            assert vs[0]["threads"] == 512 and vs[0]["cols"] == 2 and vs[0]["pad_shift"] == 4
+        assert all(v["min_waves"] >= 1 and v["pad_shift"] >= 2 for v in vs), n
+        # defaults: index 0 of each length for both integrators
+        for integ in ("RK4CN", "AB4"):
+            assert ns2d.default_variant(ns2d.PASS_ROW, n, integ) == 0
+            assert ns2d.default_variant(ns2d.PASS_COLUMN, n, integ) == 0
         rs = ns2d.rpass_variants(n)
         assert rs, n
         for v in rs:
@@ -96,4 +93,4 @@ def test_default_variant_rejects_bad_arguments():
     assert lib.ns2d_default_variant(ns2d.PASS_COLUMN, 256, 0, None) == ns2d.NS2D_ERR_INVALID
     assert idx.value == -7
     assert lib.ns2d_default_variant(ns2d.PASS_COLUMN, 4096, 1, ctypes.byref(idx)) == ns2d.NS2D_OK
-    assert idx.value == ns2d.default_variant(ns2d.PASS_COLUMN, 4096, "AB4") > 0
+    assert idx.value == ns2d.default_variant(ns2d.PASS_COLUMN, 4096, "AB4") == 0
