@@ -1181,6 +1181,10 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(256, 8, 128, 4, 3, 1, 1, 1, 0) \
   X(256, 16, 256, 2, 3, 1, 0, 1, 0) \
   X(256, 32, 512, 2, 3, 1, 0, 1, 0) \
+  X(256, 8, 512, 2, 4, 2, 0, 1, 0) \
+  X(256, 8, 512, 2, 4, 8, 0, 1, 0) \
+  X(256, 8, 256, 2, 4, 2, 0, 1, 0) \
+  X(256, 8, 256, 2, 4, 8, 0, 1, 0) \
   X(512, 2, 256, 2, 3, 1, 0, 1, 0) \
   X(512, 4, 256, 2, 3, 1, 0, 1, 0) \
   X(512, 4, 256, 2, 3, 1, 0, 1, 1) \
@@ -1197,6 +1201,9 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(512, 4, 512, 2, 3, 1, 0, 1, 0) \
   X(512, 2, 256, 2, 3, 8, 0, 1, 0) \
   X(512, 2, 256, 2, 4, 8, 0, 1, 0) \
+  X(512, 8, 512, 1, 4, 2, 0, 1, 0) \
+  X(512, 8, 512, 1, 4, 8, 0, 1, 0) \
+  X(512, 8, 512, 2, 4, 2, 0, 1, 0) \
   X(1024, 2, 512, 2, 4, 8, 0, 1, 0) \
   X(1024, 2, 512, 2, 3, 1, 0, 1, 0) \
   X(1024, 2, 256, 2, 3, 1, 0, 1, 0) \
@@ -1231,6 +1238,9 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(1024, 2, 512, 2, 4, 1, 0, 1, 0) \
   X(1024, 2, 256, 2, 3, 8, 0, 1, 0) \
   X(1024, 2, 256, 2, 4, 8, 0, 1, 0) \
+  X(1024, 8, 512, 1, 4, 2, 0, 1, 0) \
+  X(1024, 8, 512, 1, 4, 8, 0, 1, 0) \
+  X(1024, 4, 512, 1, 4, 4, 0, 1, 0) \
   X(2048, 2, 512, 1, 4, 8, 0, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 0, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 0, 1, 1) \

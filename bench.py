@@ -140,6 +140,15 @@ def ensemble_leg(steps=500, seed=2024):
         e.step(dts, steps)
         e.sync()
         t_ens = time.perf_counter() - t0
+        # the shapes the tuner chose per size class (first member of each)
+        chosen = {}
+        for i, n in enumerate(sizes):
+            if n not in chosen:
+                k, r = e.variants(i)
+                kv, rv = ns2d.kpass_variants(n)[k], ns2d.rpass_variants(n)[r]
+                chosen[n] = {"kpass": k, "rpass": r,
+                             "kpass_shape": [kv[f] for f in ("cols", "threads", "splits", "pad_shift", "line_offset")],
+                             "rpass_shape": [rv[f] for f in ("threads_per_pair", "pairs", "single_row")]}
     t_seq = 0.0
     for c, w, dt in zip(cfgs, ws, dts):
         with ns2d.Solver(c) as s:
@@ -154,7 +163,8 @@ def ensemble_leg(steps=500, seed=2024):
     return {"ensemble_members": len(sizes), "ensemble_steps_each": steps,
             "ensemble_member_steps_per_s": member_steps / t_ens,
             "sequential_member_steps_per_s": member_steps / t_seq,
-            "ensemble_vs_sequential": t_seq / t_ens}
+            "ensemble_vs_sequential": t_seq / t_ens,
+            "ensemble_shapes": {str(n): v for n, v in chosen.items()}}
 
 
 def adjoint_leg(n=512, steps=2000, ckpt=100, seed=5, fd_dirs=3):
