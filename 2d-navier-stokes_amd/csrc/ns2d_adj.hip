@@ -42,13 +42,19 @@ namespace ns2d {
   X(256, 4, 256)     \
   X(256, 4, 128)     \
   X(256, 8, 128)     \
+  X(256, 1, 64)      \
+  X(256, 1, 128)     \
   X(512, 2, 256)     \
   X(512, 2, 128)     \
   X(512, 4, 128)     \
   X(512, 4, 256)     \
+  X(512, 1, 128)     \
+  X(512, 1, 64)      \
   X(1024, 2, 512)    \
   X(1024, 2, 256)    \
   X(1024, 2, 128)    \
+  X(1024, 1, 256)    \
+  X(1024, 1, 128)    \
   X(2048, 2, 512)    \
   X(2048, 2, 256)    \
   X(4096, 2, 1024)   \

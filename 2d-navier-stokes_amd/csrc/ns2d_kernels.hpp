@@ -1154,11 +1154,16 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
 // padding but no longer with the 16-slot one).  Padding one slot per 16
 // (PS = 4) with the two columns' lines 8 slots apart (OFF = 8: adjacent lanes
 // work on the two columns, whose accesses then fall on opposite halves of
-// the banks) beats one slot per 8 with adjacent lines by 6 % at 1024 and 2048
-// and 4 % at 4096.  At 1024, 512 threads with 4 rows each beat 256 threads
-// with 8 rows (shorter per-workgroup chains); at 512, two columns per 256
-// threads beat four (-12 %), at 256 eight columns per 512 threads beat 256
-// threads (-8 %).
+// the banks) beats one slot per 8 with adjacent lines by 6 % at 2048 and 4 %
+// at 4096.  Up to 1024 every workgroup runs in the first round and the pass
+// takes as long as its busiest CU: the Ny/2+1 columns make an odd column
+// count, and with two columns per workgroup the last group lands as a third
+// workgroup on two CUs (-9 % at 1024 when it is left out); one column per
+// workgroup (KS splits of the inverse fields) spreads the work over four or
+// more workgroups per CU, so the odd one costs a quarter instead of a half:
+// 1024 one column per 256 threads, 2 splits (-7.5 %); 512 one column per
+// 128 threads, 4 splits (-5 %); 256 one column per wave, 4 splits (-38 %,
+// no workgroup barriers).
 #define NS2D_KPASS_VARIANTS(X) \
   X(8, 16, 64, 1, 3, 1, 0, 1, 0) \
   X(8, 16, 64, 1, 3, 1, 1, 1, 0) \
@@ -1172,6 +1177,7 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(128, 16, 128, 1, 3, 1, 1, 1, 0) \
   X(128, 8, 64, 2, 3, 1, 1, 1, 0) \
   X(128, 8, 128, 2, 3, 1, 1, 1, 0) \
+  X(256, 1, 64, 4, 3, 1, 0, 1, 0) \
   X(256, 8, 512, 2, 3, 1, 0, 1, 0) \
   X(256, 8, 256, 2, 3, 1, 0, 1, 0) \
   X(256, 8, 256, 2, 3, 1, 0, 1, 1) \
@@ -1185,6 +1191,10 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(256, 8, 512, 2, 4, 8, 0, 1, 0) \
   X(256, 8, 256, 2, 4, 2, 0, 1, 0) \
   X(256, 8, 256, 2, 4, 8, 0, 1, 0) \
+  X(256, 1, 64, 2, 3, 1, 0, 1, 0) \
+  X(256, 2, 128, 4, 3, 1, 0, 1, 0) \
+  X(256, 1, 128, 4, 3, 1, 0, 1, 0) \
+  X(512, 1, 128, 4, 3, 1, 0, 1, 0) \
   X(512, 2, 256, 2, 3, 1, 0, 1, 0) \
   X(512, 4, 256, 2, 3, 1, 0, 1, 0) \
   X(512, 4, 256, 2, 3, 1, 0, 1, 1) \
@@ -1204,6 +1214,10 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(512, 8, 512, 1, 4, 2, 0, 1, 0) \
   X(512, 8, 512, 1, 4, 8, 0, 1, 0) \
   X(512, 8, 512, 2, 4, 2, 0, 1, 0) \
+  X(512, 1, 128, 2, 3, 1, 0, 1, 0) \
+  X(512, 1, 64, 4, 3, 1, 0, 1, 0) \
+  X(512, 1, 64, 2, 3, 1, 0, 1, 0) \
+  X(1024, 1, 256, 2, 3, 1, 0, 1, 0) \
   X(1024, 2, 512, 2, 4, 8, 0, 1, 0) \
   X(1024, 2, 512, 2, 3, 1, 0, 1, 0) \
   X(1024, 2, 256, 2, 3, 1, 0, 1, 0) \
@@ -1241,6 +1255,9 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(1024, 8, 512, 1, 4, 2, 0, 1, 0) \
   X(1024, 8, 512, 1, 4, 8, 0, 1, 0) \
   X(1024, 4, 512, 1, 4, 4, 0, 1, 0) \
+  X(1024, 1, 256, 2, 4, 1, 0, 1, 0) \
+  X(1024, 1, 256, 4, 4, 1, 0, 1, 0) \
+  X(1024, 1, 128, 2, 4, 1, 0, 1, 0) \
   X(2048, 2, 512, 1, 4, 8, 0, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 0, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 0, 1, 1) \
@@ -1253,6 +1270,10 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(2048, 2, 512, 1, 3, 1, 2, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 2, 4, 0) \
   X(2048, 2, 512, 1, 3, 8, 0, 1, 0) \
+  X(2048, 1, 256, 1, 3, 1, 0, 1, 0) \
+  X(2048, 1, 256, 2, 3, 1, 0, 1, 0) \
+  X(2048, 1, 128, 2, 3, 1, 0, 1, 0) \
+  X(2048, 1, 512, 1, 3, 1, 0, 1, 0) \
   X(4096, 2, 512, 1, 4, 8, 0, 1, 0) \
   X(4096, 2, 1024, 1, 3, 4, 1, 1, 0) \
   X(4096, 2, 512, 1, 3, 1, 1, 1, 0) \
@@ -1268,7 +1289,9 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(4096, 1, 512, 2, 3, 1, 1, 1, 0) \
   X(4096, 2, 512, 1, 3, 1, 0, 1, 0) \
   X(4096, 2, 1024, 1, 4, 8, 1, 1, 0) \
-  X(4096, 2, 512, 1, 3, 8, 0, 1, 0)
+  X(4096, 2, 512, 1, 3, 8, 0, 1, 0) \
+  X(4096, 1, 256, 1, 4, 1, 0, 1, 0) \
+  X(4096, 1, 512, 1, 4, 1, 0, 1, 0)
 
 struct KVariant {
   int L, C, T, KS, PS, OFF, PK, W, PF;
@@ -1322,6 +1345,9 @@ struct KVariant {
   X(1024, 64, 4, 1, 0) \
   X(1024, 128, 1, 1, 2) \
   X(1024, 256, 1, 1, 3) \
+  X(1024, 64, 1, 1, 2) \
+  X(1024, 64, 1, 1, 3) \
+  X(1024, 128, 1, 2, 2) \
   X(2048, 256, 1, 1, 0) \
   X(2048, 128, 1, 2, 0) \
   X(2048, 128, 1, 1, 0) \
