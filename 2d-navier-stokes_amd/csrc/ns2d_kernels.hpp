@@ -1178,6 +1178,7 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(128, 8, 64, 2, 3, 1, 1, 1, 0) \
   X(128, 8, 128, 2, 3, 1, 1, 1, 0) \
   X(256, 1, 64, 4, 3, 1, 0, 1, 0) \
+  X(256, 1, 64, 4, 3, 1, 0, 1, 1) \
   X(256, 8, 512, 2, 3, 1, 0, 1, 0) \
   X(256, 8, 256, 2, 3, 1, 0, 1, 0) \
   X(256, 8, 256, 2, 3, 1, 0, 1, 1) \
@@ -1195,6 +1196,7 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(256, 2, 128, 4, 3, 1, 0, 1, 0) \
   X(256, 1, 128, 4, 3, 1, 0, 1, 0) \
   X(512, 1, 128, 4, 3, 1, 0, 1, 0) \
+  X(512, 1, 128, 4, 3, 1, 0, 1, 1) \
   X(512, 2, 256, 2, 3, 1, 0, 1, 0) \
   X(512, 4, 256, 2, 3, 1, 0, 1, 0) \
   X(512, 4, 256, 2, 3, 1, 0, 1, 1) \
@@ -1218,6 +1220,7 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(512, 1, 64, 4, 3, 1, 0, 1, 0) \
   X(512, 1, 64, 2, 3, 1, 0, 1, 0) \
   X(1024, 1, 256, 2, 3, 1, 0, 1, 0) \
+  X(1024, 1, 256, 2, 3, 1, 0, 1, 1) \
   X(1024, 2, 512, 2, 4, 8, 0, 1, 0) \
   X(1024, 2, 512, 2, 3, 1, 0, 1, 0) \
   X(1024, 2, 256, 2, 3, 1, 0, 1, 0) \

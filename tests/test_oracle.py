@@ -195,6 +195,32 @@ def test_fourth_order_in_time(integ):
     assert 13.0 < r < 19.0, r
 
 
+def test_euler_build_is_rk4cn_without_viscosity():
+    """The reference built with -D__EULER (inviscid) takes the explicit update
+    w + (dt/6)(K1 + 2K2 + 2K3 + K4) (solver.c:827-830) after the same four
+    stages; the RK4CN update with nu = alpha = 0 (D = 0: factors 1 and dt)
+    is that step up to the association of the sums, so Euler runs are RK4CN
+    runs with zero viscosity and drag."""
+    N = 32
+    p = O.Params(N, N, nu=0.0, hyper_flag=0, hyper_pow=1.0, filter="23")
+    w0 = O.ic_random_spectrum(p, seed=11, k0=4.0, energy=0.5)
+    dt = 1e-3
+    st = O.State(w0, p)
+    O.rk4cn_step(st, dt)
+    # the explicit update from the same stages (quirks as in rk4cn_step)
+    st2 = O.State(w0, p)
+    w = st2.w_hat
+    K1 = O.rhs_with_side_effect(w, w, p)
+    K2 = O.rhs_with_side_effect(w + (dt * 0.5) * K1, w, p)
+    K3 = O.rhs_with_side_effect(w + (dt * 0.5) * K2, w, p)
+    K4 = O.rhs_with_side_effect(w + dt * K3, w, p)
+    euler = w + (dt / 6.0) * (K1 + 2.0 * K2 + 2.0 * K3 + K4)
+    euler[0, 0] = 0.0
+    err = np.linalg.norm(st.w_hat - euler) / np.linalg.norm(euler)
+    assert err < 1e-15, err
+    assert np.linalg.norm(euler - w0) > 1e-6 * np.linalg.norm(w0)   # the step does something
+
+
 @pytest.mark.parametrize("integ,lo,hi", [("RK4CN", 1.7, 2.4), ("AB4", 3.4, 4.6)])
 def test_reference_conjugacy_write_lowers_order(integ, lo, hi):
     """With the reference's ForceConjugacy write into the global w_hat (Q2,
