@@ -1161,7 +1161,8 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
 // workgroup on two CUs (-9 % at 1024 when it is left out); one column per
 // workgroup (KS splits of the inverse fields) spreads the work over four or
 // more workgroups per CU, so the odd one costs a quarter instead of a half:
-// 1024 one column per 256 threads, 2 splits (-7.5 %); 512 one column per
+// 1024 one column per 256 threads, 2 splits, 16-slot padding (-7.5 %, and
+// -0.3 % per step against 8-slot padding in interleaved timing); 512 one column per
 // 128 threads, 4 splits (-5 %); 256 one column per wave, 4 splits (-38 %,
 // no workgroup barriers).
 #define NS2D_KPASS_VARIANTS(X) \
@@ -1219,6 +1220,7 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(512, 1, 128, 2, 3, 1, 0, 1, 0) \
   X(512, 1, 64, 4, 3, 1, 0, 1, 0) \
   X(512, 1, 64, 2, 3, 1, 0, 1, 0) \
+  X(1024, 1, 256, 2, 4, 1, 0, 1, 0) \
   X(1024, 1, 256, 2, 3, 1, 0, 1, 0) \
   X(1024, 1, 256, 2, 3, 1, 0, 1, 1) \
   X(1024, 2, 512, 2, 4, 8, 0, 1, 0) \
@@ -1258,7 +1260,6 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(1024, 8, 512, 1, 4, 2, 0, 1, 0) \
   X(1024, 8, 512, 1, 4, 8, 0, 1, 0) \
   X(1024, 4, 512, 1, 4, 4, 0, 1, 0) \
-  X(1024, 1, 256, 2, 4, 1, 0, 1, 0) \
   X(1024, 1, 256, 4, 4, 1, 0, 1, 0) \
   X(1024, 1, 128, 2, 4, 1, 0, 1, 0) \
   X(2048, 2, 512, 1, 4, 8, 0, 1, 0) \
@@ -1309,8 +1310,11 @@ struct KVariant {
 // PFR = 1 loads both rows' G fields up front (shapes with E = 4), PFR = 2 runs
 // one row per TLR threads instead of a pair, PFR = 3 one row with its four G
 // fields loaded before the first transform; the first listed for each NY is
-// the default.  Single rows win at 256 to 1024 (these grids run in one round
-// and are bound by a workgroup's chain of transforms) and at 4096, where the
+// the default.  Single rows win at 256 and 1024 (these grids run in one round
+// and are bound by a workgroup's chain of transforms; at 1024 one wave per
+// row with all loads up front, no workgroup barriers: -0.6 % per step in
+// interleaved timing against 256 threads; at 512 one-wave row pairs beat
+// single rows by 1.1 %) and at 4096, where the
 // mixed-space fields' 2-row tiles make a row pair's sequential halves read
 // each 32-B run twice (single rows 260 against 287 us per launch, the two
 // rows of a tile on one XCD); at 2048 256-thread row pairs (64 against 74 us).
@@ -1330,8 +1334,8 @@ struct KVariant {
   X(256, 16, 4, 1, 0) \
   X(256, 32, 2, 1, 0) \
   X(256, 64, 1, 1, 3) \
-  X(512, 64, 1, 1, 2) \
   X(512, 64, 1, 1, 0) \
+  X(512, 64, 1, 1, 2) \
   X(512, 32, 2, 1, 0) \
   X(512, 128, 1, 1, 0) \
   X(512, 128, 1, 1, 1) \
@@ -1339,6 +1343,7 @@ struct KVariant {
   X(512, 64, 4, 1, 0) \
   X(512, 128, 1, 1, 2) \
   X(512, 64, 1, 1, 3) \
+  X(1024, 64, 1, 1, 3) \
   X(1024, 256, 1, 1, 2) \
   X(1024, 256, 1, 1, 0) \
   X(1024, 256, 1, 1, 1) \
@@ -1349,7 +1354,6 @@ struct KVariant {
   X(1024, 128, 1, 1, 2) \
   X(1024, 256, 1, 1, 3) \
   X(1024, 64, 1, 1, 2) \
-  X(1024, 64, 1, 1, 3) \
   X(1024, 128, 1, 2, 2) \
   X(2048, 256, 1, 1, 0) \
   X(2048, 128, 1, 2, 0) \

@@ -1,4 +1,5 @@
 """CPU-side checks of the C ABI and the host package (no GPU calls)."""
+import ctypes
 import os
 import re
 
@@ -51,7 +52,7 @@ def test_variant_tables_cover_every_size():
             assert v["threads"] == v["threads_per_pair"] * v["pairs"]
             assert v["threads_per_pair"] > 64 and v["pairs"] == 1 or 64 % v["threads_per_pair"] == 0
             assert not (v["single_row"] and v["pairs"] != 1), n
-        assert rs[0]["single_row"] == (n in (256, 512, 1024, 4096)), n
+        assert rs[0]["single_row"] == (n in (256, 1024, 4096)), n
         if n >= 256:
             assert any(v["single_row"] for v in rs), n
 
@@ -77,6 +78,23 @@ def test_invalid_params_rejected_without_gpu():
     rc = lib.ns2d_create(ctypes.byref(p), ctypes.byref(h))
     assert rc == ns2d.NS2D_ERR_INVALID
     assert b"powers of two" in lib.ns2d_last_error()
+
+
+def test_layout_query_and_traffic_key():
+    """ns2d_layout reports the compiled device layout without a GPU (k-space
+    column-major, 2-row x-tiles for both mixed-space sets by default), rejects
+    null pointers, and bench.py keys its traffic table by kernel signature
+    and this layout."""
+    import ns2d
+    lib = ns2d.load_library()
+    assert ns2d.layout() == (0, 2, 2)
+    k = ctypes.c_int(-1)
+    assert lib.ns2d_layout(ctypes.byref(k), None, None) == ns2d.NS2D_ERR_INVALID
+    import bench
+    key = bench.traffic_key("kpass", 1024)
+    assert key.startswith("kpass_kernel<1024, ") and key.endswith(" layout k0g2n2")
+    v = ns2d.kpass_variants(1024)[ns2d.default_variant(ns2d.PASS_COLUMN, 1024)]
+    assert f"<1024, {v['cols']}, {v['threads']}, {v['splits']}, {v['pad_shift']}, {v['line_offset']}, " in key
 
 
 def test_default_variant_rejects_bad_arguments():
