@@ -165,6 +165,32 @@ __device__ __forceinline__ double2 tw_get(const TwReg<N, TL>& t, int, int slot) 
   return SIGN < 0 ? w : make_double2(w.x, -w.y);
 }
 
+// LDS slot of line position p in the exchange after a pass of span P and
+// radix R.  Padded (one slot every 2^PS), or, with NS2D_FFT_SWIZZLE, XOR-
+// swizzled within rows of 16 slots (256 B, the 64 banks' width): sixteen
+// consecutive lanes write the positions a*P*R + k + r*P (k < P, a < 16/P) of
+// their 16/P blocks, which a pass with P < 16 puts on the same bank columns
+// of different rows; XORing a row's columns with P * (its block number mod
+// 16/P) gives every lane of the sixteen its own column, and the reads of 16
+// consecutive positions stay in one row (a permutation of its columns).  The
+// slot never leaves its row, so a line needs no more than N slots.
+#ifndef NS2D_FFT_SWIZZLE
+#define NS2D_FFT_SWIZZLE 0
+#endif
+template <int PS, int P, int R>
+__device__ __forceinline__ int fft_slot(int p) {
+  if constexpr (!NS2D_FFT_SWIZZLE) {
+    return padx_s<PS>(p);
+  } else if constexpr (P >= 16) {
+    return p;   // sixteen lanes, sixteen consecutive positions
+  } else {
+    constexpr int PR = P * R;
+    constexpr int RS = PR >= 16 ? PR / 16 : 1;   // rows per block
+    constexpr int M = 16 / P;                    // blocks under sixteen lanes
+    return p ^ ((((p >> 4) / RS) & (M - 1)) * P);
+  }
+}
+
 template <int N, int TL, int SIGN, bool BLOCK, int P, int PS, int OFF, typename TW>
 __device__ __forceinline__ void fft_reg_pass(double2* v, double2* buf, const TW& tw, int l) {
   constexpr int E = N / TL;
@@ -190,13 +216,13 @@ __device__ __forceinline__ void fft_reg_pass(double2* v, double2* buf, const TW&
       } else {
         const int j0 = (i - k) * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) buf[padx_s<PS>(j0 + r * P)] = x[r];
+        for (int r = 0; r < R; ++r) buf[fft_slot<PS, P, R>(j0 + r * P)] = x[r];
       }
     }
     if constexpr (!LAST) {
       xsync<BLOCK>();
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = buf[padx_s<PS>(l + TL * e)];
+      for (int e = 0; e < E; ++e) v[e] = buf[fft_slot<PS, P, R>(l + TL * e)];
       xsync<BLOCK>();
       fft_reg_pass<N, TL, SIGN, BLOCK, P * R, PS, OFF + (P > 1 ? NB : 0)>(v, buf, tw, l);
     }

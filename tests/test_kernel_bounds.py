@@ -318,3 +318,48 @@ def test_spectral_tile_index_is_a_permutation(xt):
             ky = np.arange(ld)[None, :]
             idx = (((x >> s) * ld + ky) << s) + (x & (xt - 1))
             assert np.array_equal(np.sort(idx.ravel()), np.arange(nx * ld)), (nx, ny)
+
+
+def fft_slot_swizzled(p, P, R):
+    """fft_slot with NS2D_FFT_SWIZZLE (ns2d_fft_reg.hpp)."""
+    if P >= 16:
+        return p
+    PR = P * R
+    RS = PR // 16 if PR >= 16 else 1
+    M = 16 // P
+    return p ^ ((((p >> 4) // RS) & (M - 1)) * P)
+
+
+def test_fft_swizzle_is_a_row_permutation_without_bank_conflicts():
+    """The optional XOR swizzle of the register FFT's LDS exchanges keeps
+    every slot in its 16-slot row (a line needs no more than N slots) and is
+    one to one; in a 64-bank, 16-B-slot model every group of sixteen lanes
+    writes (Stockham positions of each pass with span P < 16) and reads
+    (sixteen consecutive positions) sixteen different bank columns, for every
+    line length and lane count the kernels use."""
+    for N in LENGTHS:
+        for TL in (16, 32, 64, 128, 256, 512, 1024):
+            if TL > N or N // TL < 2:
+                continue
+            E = N // TL
+            P = 1
+            while N // P > 1:
+                R = min(min(E, 16), N // P)
+                if P * R == N:
+                    break   # the last pass ends in registers
+                perm = [fft_slot_swizzled(p, P, R) for p in range(N)]
+                assert sorted(perm) == list(range(N)), (N, TL, P, R)
+                assert all(a >> 4 == b >> 4 for a, b in zip(perm, range(N)))
+                for m in range(E // R):
+                    for r in range(R):
+                        for g in range(0, TL, 16):
+                            cols = set()
+                            for t in range(g, g + 16):
+                                i = t + TL * m
+                                k = i & (P - 1)
+                                cols.add(fft_slot_swizzled((i - k) * R + k + r * P, P, R) % 16)
+                            assert len(cols) == 16, (N, TL, P, R, m, r)
+                for e in range(E):
+                    for g in range(0, TL, 16):
+                        assert len({fft_slot_swizzled(t + TL * e, P, R) % 16 for t in range(g, g + 16)}) == 16
+                P *= R
