@@ -178,20 +178,40 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
   // (every epilogue that reads the state runs after a forward transform on a
   // valid column, exactly when `pre` holds; the arrays are filled
   // unconditionally so that they stay in registers)
-  const bool pre = PF && (a.mode & KP_FWD) && colv;
+  const bool pfw = PF != 0;
+  const bool pre = pfw && (a.mode & KP_FWD) && colv;
   if constexpr (PF != 0) {
-    {
+#pragma unroll
+    for (int it = 0; it < E; ++it) {
+      wpre[it] = apre[it] = make_double2(0.0, 0.0);
+      lpre[it] = 0.0;
+    }
+    const bool epi_rk = a.epi == EPI_RK1 || a.epi == EPI_RK2 || a.epi == EPI_RK3 || a.epi == EPI_RK4CN ||
+                        a.epi == EPI_AB4;
+    if (pfw && epi_rk && (a.mode & KP_FWD)) {   // uniform per workgroup
       const bool cf = (a.mode & KP_CONJFIX) && ky == 0;
       const bool need_acc = a.epi == EPI_RK4CN || ((a.epi == EPI_RK2 || a.epi == EPI_RK3) && writer);
       const bool need_lam = a.epi == EPI_RK4CN;
+      // every row's loads issued without a branch around them: a slot past the
+      // last column reads column 0 (in bounds) and keeps its zeros
+      const int kyl = colv ? ky : 0;
 #pragma unroll
       for (int it = 0; it < E; ++it) {
         const int i = l + TL * it;
         const int kx = kx_of_row(i, nx);
-        const size_t idx = kidx(i, ky, ld, nx);
-        wpre[it] = !pre ? make_double2(0.0, 0.0) : (cf && kx < 0) ? cconj(a.conj_src[-kx]) : a.w_in[idx];
-        apre[it] = (pre && need_acc) ? a.acc[idx] : make_double2(0.0, 0.0);
-        lpre[it] = (pre && need_lam) ? a.lam[idx] : 0.0;
+        const size_t idx = kidx(i, kyl, ld, nx);
+        const bool cj = cf && kx < 0;
+        const double2* pw = cj ? a.conj_src + (cj ? -kx : 0) : a.w_in + idx;
+        const double2 w = *pw;
+        if (pre) wpre[it] = make_double2(w.x, cj ? -w.y : w.y);
+        if (need_acc) {
+          const double2 c = a.acc[idx];
+          if (pre) apre[it] = c;
+        }
+        if (need_lam) {
+          const double m = a.lam[idx];
+          if (pre) lpre[it] = m;
+        }
       }
     }
   }
