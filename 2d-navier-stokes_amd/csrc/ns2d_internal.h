@@ -122,6 +122,9 @@ struct KPassArgs {
   int nyf, ld;
   const double* lam;     // per-mode D_fac/dt of the CN update, [Nx][ld]
   int mode, epi;
+  // 1: the last column group (one valid column, KS = 1 shapes) runs as two
+  // work items with two of the four inverse fields each (set by the launcher)
+  int last_split;
   double dt, coef;
   PhysArgs ph;
 };

@@ -140,7 +140,12 @@ hipError_t launch_kpass(int nx, int variant, const KPassArgs& a, hipStream_t s) 
     v = w;
   }
   const int groups = v->groups(a.nyf);
-  hipLaunchKernelGGL(v->kern, dim3(groups * v->KS), dim3(v->T), 0, s, a);
+  // A last column group holding a single column (unpacked, KS = 1, more than
+  // one column per group) runs as two half items: it would otherwise be the
+  // one workgroup left when every CU has finished its share
+  KPassArgs b = a;
+  b.last_split = (v->KS == 1 && v->PK == 0 && v->C > 1 && (a.mode & KP_INV) && a.nyf % v->C == 1) ? 1 : 0;
+  hipLaunchKernelGGL(v->kern, dim3(groups * v->KS + b.last_split), dim3(v->T), 0, s, b);
   return hipGetLastError();
 }
 

@@ -145,7 +145,13 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
   __shared__ double2 lds[C * LS];
 
   const int tid = threadIdx.x;
-  const int g = wi / KS, split = wi % KS;
+  // with last_split (KS = 1 shapes), work items ngroups-1 and ngroups are the
+  // last column group's two halves: two inverse fields each, split 0 writes
+  const int ngroups = ((PK ? a.nyf - 1 : a.nyf) + C - 1) / C;
+  const bool tail = KS == 1 && a.last_split && wi >= ngroups - 1;
+  const int g = tail ? ngroups - 1 : wi / KS;
+  const int split = tail ? wi - (ngroups - 1) : wi % KS;
+  const int nfs = tail ? 2 : FPS;   // inverse fields of this work item
   const int c = tid % C, l = tid / C;
   const int ky = g * C + c;        // the slot; with PK slot 0 is the (0, Ny/2) pair
   const int nyf = a.nyf, ld = a.ld, nx = L;
@@ -449,8 +455,8 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
     return make_double2(kx * inv * s.y, -(kx * inv) * s.x);
   };
 #pragma unroll 1
-  for (int fi = 0; fi < FPS; ++fi) {
-    const int f = split * FPS + fi;
+  for (int fi = 0; fi < nfs; ++fi) {
+    const int f = split * nfs + fi;
     const int l_o = opaque(l), ky_o = opaque(ky);
     double2 F[E];
     if (pblk) {

@@ -138,6 +138,28 @@ def test_column_pass_indices(L):
             # ForceConjugacy hand-off: conj_src[-kx] for kx < 0, conj_dst[x]
             kx = np.where(rows <= L // 2, rows, rows - L)
             assert (-kx[kx < 0]).max(initial=0) < L and rows.max() < L
+            # single-solver launch with last_split (launch_kpass: unpacked,
+            # KS = 1, C > 1, one valid column in the last group): the grid has
+            # one more work item; items groups-1 and groups are the last
+            # group's halves with inverse fields {0, 1} and {2, 3}, split 0
+            # the only writer, every other item unchanged
+            if KS == 1 and PK == 0 and C > 1 and nyf % C == 1:
+                n = groups + 1
+                wi = np.arange(n)
+                tail = wi >= groups - 1
+                g2 = np.where(tail, groups - 1, wi)
+                s2 = np.where(tail, wi - (groups - 1), 0)
+                nf = np.where(tail, 2, 4)
+                assert g2.max() == groups - 1 and s2.max() == 1
+                f_lo, f_hi = s2 * nf, s2 * nf + nf - 1
+                assert f_lo.min() == 0 and f_hi.max() == 3
+                last = wi[tail]
+                assert sorted((s2[last] * 2).tolist() + (s2[last] * 2 + 1).tolist()) == [0, 1, 2, 3]
+                ky2 = g2[:, None] * C + np.arange(C)[None, :]
+                own = np.sort(ky2[s2 == 0][(ky2 < nslots)[s2 == 0]])
+                assert np.array_equal(own, np.arange(nslots)), (v, ny)
+                b = xcd_remap(np.arange(n), n)
+                assert np.array_equal(np.sort(b), np.arange(n))
             # batched launch: b -> xcd_remap -> member m = b // per
             for members in (1, 3, 24):
                 n = grid * members
