@@ -211,8 +211,8 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
         const double2 w = *pw;
         if (pre) wpre[it] = make_double2(w.x, cj ? -w.y : w.y);
         if (need_acc) {
-          const double2 c = a.acc[idx];
-          if (pre) apre[it] = c;
+          const double2 av = a.acc[idx];
+          if (pre) apre[it] = av;
         }
         if (need_lam) {
           const double m = a.lam[idx];
