@@ -64,6 +64,7 @@ void ns2d_cli_defaults(ns2d_run_cfg* c) {
   copy_str(c->filter, sizeof c->filter, "23");
   c->quirks = NS2D_QUIRKS_REF;
   c->testing = 0;
+  c->ic_ranks = 1;
 }
 
 /* initial conditions the reference accepts for -i (utils.c:262-345); any
@@ -82,7 +83,7 @@ int ns2d_cli_parse(int argc, char** argv, ns2d_run_cfg* c, char* msg, size_t n) 
   int opt;
   optind = 1;
   opterr = 0;
-  while ((opt = getopt(argc, argv, "o:h:n:d:s:e:t:v:i:c:p:f:z:T:I:D:Q:X")) != -1) {
+  while ((opt = getopt(argc, argv, "o:h:n:d:s:e:t:v:i:c:p:f:z:T:I:D:Q:R:X")) != -1) {
     switch (opt) {
       case 'o':
         if (out_i == 0) copy_str(c->output_dir, sizeof c->output_dir, optarg);
@@ -245,6 +246,14 @@ int ns2d_cli_parse(int argc, char** argv, ns2d_run_cfg* c, char* msg, size_t n) 
         c->quirks = (unsigned)q;
         break;
       }
+      case 'R': {
+        char* end = NULL;
+        long p = strtol(optarg, &end, 10);
+        if (!end || *end || p < 1 || p > NS2D_IC_MAX_RANKS)
+          return say(msg, n, PFX "The rank count [%s] must be an integer in 1..%d\n", optarg, NS2D_IC_MAX_RANKS);
+        c->ic_ranks = (int)p;
+        break;
+      }
       case 'X':
         c->testing = 1;
         break;
@@ -265,7 +274,8 @@ int ns2d_cli_parse(int argc, char** argv, ns2d_run_cfg* c, char* msg, size_t n) 
                    "Use -p to specify how often to print to file\n"
                    "Use -z to specify an input file to read parameters from\n"
                    "Use -I / -D / -Q to select the integrator (RK4CN, AB4), the dealiasing filter (23, HOU_LI)\n"
-                   "and the reference-quirk mask; -X writes the Taylor-Green solution and error norms\n"
+                   "and the reference-quirk mask; -X writes the Taylor-Green solution and error norms;\n"
+                   "-R P builds the initial condition of a run on P MPI ranks\n"
                    "-->> Now Exiting!\n");
     }
   }

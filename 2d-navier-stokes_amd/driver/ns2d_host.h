@@ -16,7 +16,9 @@ extern "C" {
  * reference's defaults (utils.c:48-88), plus the choices the reference makes
  * at compile time (data_types.h:49-60, Solver/Makefile DSET_FLAGS), which this
  * driver takes as extra flags: -I integrator, -D filter, -Q quirk mask,
- * -X (the TESTING build's Taylor-Green solution and error norms). */
+ * -X (the TESTING build's Taylor-Green solution and error norms), and the
+ * process count of `mpirun -n P` (run_solver.py:270), on which the reference's
+ * random initial conditions depend: -R P. */
 typedef struct ns2d_run_cfg {
   char input_file_name[512];
   char input_dir[512];
@@ -46,7 +48,11 @@ typedef struct ns2d_run_cfg {
   char filter[16];     /* "23" (__DEALIAS_23, default) or "HOU_LI"            */
   unsigned quirks;     /* NS2D_QUIRK_* bits, default NS2D_QUIRKS_REF            */
   int testing;         /* -X: TGSoln dataset and L2/Linf print (TESTING build) */
+  int ic_ranks;        /* -R P: the initial condition of a P-rank run, default 1 */
 } ns2d_run_cfg;
+
+/* the largest rank count -R accepts */
+#define NS2D_IC_MAX_RANKS 65536
 
 void ns2d_cli_defaults(ns2d_run_cfg* c);
 

@@ -4,7 +4,9 @@
  * the whole grid: the random draws come from glibc rand() after
  * srand(123456789) (solver.c:1121), consumed in the reference's (i, j) loop
  * order and under the same conditions, so a run reproduces the reference's
- * single-rank field.  The arithmetic of each formula is kept in the
+ * single-rank field, or with ic_ranks = P the field of a P-rank run (each
+ * rank's generator restarted on its own slab, the normalisation sums reduced
+ * over the slabs; ic_slabs below).  The arithmetic of each formula is kept in the
  * reference's operation order (C99 complex, no FMA contraction on x86-64) so
  * that the spectral initial conditions agree bit for bit; the ones built in
  * real space go through the caller's transforms.
@@ -59,6 +61,52 @@ static int kx_of(long i, long Nx) { return (int)(i <= Nx / 2 ? i : i - Nx); }
 static double x_of(long i, long N) { return (double)i * 2.0 * M_PI / (double)N; }
 
 static double urand(void) { return (double)rand() / (double)RAND_MAX; }
+
+/* The MPI ranks of the reference run being reproduced.  Every rank calls
+ * srand(123456789) (solver.c:1121) and walks only its own slab of kx rows, so
+ * a random initial condition depends on the rank count.  The slabs are
+ * FFTW's default block distribution (fftw_mpi_local_size_many with
+ * FFTW_MPI_DEFAULT_BLOCK, solver.c:2615): ceil(Nx/P) rows per rank, the last
+ * ranks short or empty.  A normalisation sum is formed per slab, from zero,
+ * and the partial sums combined in the order of MPI_Allreduce's recursive
+ * doubling for one double (the short-message algorithm of MPICH and Open
+ * MPI): ranks below twice the excess over the largest power of two P2 <= P
+ * first add in pairs (0+1, 2+3, ...), then the P2 values add as a pairwise
+ * tree.  With P = 1 every generator is the single sequential walk. */
+typedef struct ic_slabs {
+  int P;
+  long rows;    /* kx rows per rank */
+  double* part; /* one partial sum per rank */
+} ic_slabs;
+
+/* reseed where rank i / rows starts its walk, then advance the generator by
+ * the draws that rank made before this loop */
+static void slab_reseed(const ic_slabs* s, long i, long skip) {
+  if (i % s->rows) return;
+  srand(IC_SEED);
+  for (long q = 0; q < skip; ++q) (void)rand();
+}
+
+static void slab_clear(ic_slabs* s) {
+  for (int r = 0; r < s->P; ++r) s->part[r] = 0.0;
+}
+
+static double* slab_part(ic_slabs* s, long i) { return &s->part[i / s->rows]; }
+
+/* rank i / rows's rows: the draws one draw per mode leaves behind */
+static long slab_len(const ic_slabs* s, long i, long Nx) { return Nx - i < s->rows ? Nx - i : s->rows; }
+
+static double slab_allreduce(ic_slabs* s) {
+  double* b = s->part;
+  int p2 = 1;
+  while (p2 * 2 <= s->P) p2 *= 2;
+  const int rem = s->P - p2;
+  /* every read index is at or above the index written, so in place is safe */
+  for (int r = 0; r < p2; ++r) b[r] = r < rem ? b[2 * r] + b[2 * r + 1] : b[r + rem];
+  for (int h = 1; h < p2; h *= 2)
+    for (int r = 0; r + h < p2; r += 2 * h) b[r] += b[r + h];
+  return b[0];
+}
 
 void ns2d_host_dealias(const ns2d_run_cfg* c, double* f_hat, int dim) {
   const long Nx = c->N[0], Ny = c->N[1], Nyf = Ny / 2 + 1;
@@ -182,11 +230,12 @@ static void ic_zero(const ns2d_run_cfg* c, double complex* w) {
   }
 }
 
-static void ic_decay_turb(const ns2d_run_cfg* c, double complex* w) {
+static void ic_decay_turb(const ns2d_run_cfg* c, double complex* w, ic_slabs* sl) {
   const long Nx = c->N[0], Ny = c->N[1], Nyf = Ny / 2 + 1;
   const int bb = !strcmp(c->u0, "DECAY_TURB_BB"), nb = !strcmp(c->u0, "DECAY_TURB_NB");
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
+    slab_reseed(sl, i, 0);
     for (long j = 0; j < Nyf; ++j) {
       const int k1 = (int)j;
       if (k0 == 0 && k1 == 0) {
@@ -203,20 +252,23 @@ static void ic_decay_turb(const ns2d_run_cfg* c, double complex* w) {
       else w[i * Nyf + j] = sqrt(spec_1d * (sqrt_k / (2.0 * M_PI))) * cexp(2.0 * M_PI * u1 * I);
     }
   }
-  double enrg = 0.0;
+  slab_clear(sl);
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
+    double* enrg = slab_part(sl, i);
     for (long j = 0; j < Nyf; ++j) {
       const int k1 = (int)j;
       if (k0 != 0 || k1 != 0) {
         const double inv_k_sqr = 1.0 / (double)(k0 * k0 + k1 * k1);
         const double complex z = w[i * Nyf + j];
-        if (j == 0 || j == Nyf - 1) enrg += inv_k_sqr * cabs(z * conj(z));
-        else enrg += 2.0 * inv_k_sqr * cabs(z * conj(z));
+        if (j == 0 || j == Nyf - 1) *enrg += inv_k_sqr * cabs(z * conj(z));
+        else *enrg += 2.0 * inv_k_sqr * cabs(z * conj(z));
       }
     }
   }
-  enrg *= (0.5 / pow(Nx * Ny, 2.0)) * 4.0 * pow(M_PI, 2.0);
+  /* solver.c:1331-1334: each rank scales its own sum before the reduction */
+  for (int r = 0; r < sl->P; ++r) sl->part[r] *= (0.5 / pow(Nx * Ny, 2.0)) * 4.0 * pow(M_PI, 2.0);
+  const double enrg = slab_allreduce(sl);
   const double e0 = bb ? IC_DT_E0 : nb ? IC_DT2_E0 : IC_DTEXP_E0;
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
@@ -229,8 +281,8 @@ static void ic_decay_turb(const ns2d_run_cfg* c, double complex* w) {
   }
 }
 
-static int ic_decay_turb_alt(const ns2d_run_cfg* c, double complex* w, const ns2d_transforms* tf, char* msg,
-                             size_t n) {
+static int ic_decay_turb_alt(const ns2d_run_cfg* c, double complex* w, const ns2d_transforms* tf, ic_slabs* sl,
+                             char* msg, size_t n) {
   const long Nx = c->N[0], Ny = c->N[1], Nyf = Ny / 2 + 1;
   double complex* psi_hat = (double complex*)malloc(sizeof(double complex) * Nx * Nyf);
   double* psi = (double*)malloc(sizeof(double) * Nx * Ny);
@@ -241,6 +293,7 @@ static int ic_decay_turb_alt(const ns2d_run_cfg* c, double complex* w, const ns2
   }
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
+    slab_reseed(sl, i, 0);
     for (long j = 0; j < Nyf; ++j) {
       const int k1 = (int)j;
       const double u1 = urand();
@@ -262,11 +315,16 @@ static int ic_decay_turb_alt(const ns2d_run_cfg* c, double complex* w, const ns2
     free(psi);
     return say(msg, n, "\n[ERROR] --- inverse transform of the stream function failed\n");
   }
-  double mean = 0.0;
-  for (long q = 0; q < Nx * Ny; ++q) {
-    psi[q] /= (Nx * Ny);
-    mean += psi[q];
+  /* real-space rows are slabbed as the kx rows are */
+  slab_clear(sl);
+  for (long i = 0; i < Nx; ++i) {
+    double* mean = slab_part(sl, i);
+    for (long j = 0; j < Ny; ++j) {
+      psi[i * Ny + j] /= (Nx * Ny);
+      *mean += psi[i * Ny + j];
+    }
   }
+  const double mean = slab_allreduce(sl);
   for (long q = 0; q < Nx * Ny; ++q) psi[q] -= (mean / (Nx * Ny));
   int rc = tf->r2c(tf->ctx, psi, (double*)psi_hat);
   free(psi);
@@ -274,18 +332,21 @@ static int ic_decay_turb_alt(const ns2d_run_cfg* c, double complex* w, const ns2
     free(psi_hat);
     return say(msg, n, "\n[ERROR] --- forward transform of the stream function failed\n");
   }
-  double enrg = 0.0;
+  slab_clear(sl);
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
+    double* enrg = slab_part(sl, i);
     for (long j = 0; j < Nyf; ++j) {
       const int k1 = (int)j;
       const double k_sqrd = (double)(k0 * k0 + k1 * k1);
       const double complex z = psi_hat[i * Nyf + j];
-      if (j == 0 || j == Nyf - 1) enrg += k_sqrd * cabs(z * conj(z));
-      else enrg += 2.0 * k_sqrd * cabs(z * conj(z));
+      if (j == 0 || j == Nyf - 1) *enrg += k_sqrd * cabs(z * conj(z));
+      else *enrg += 2.0 * k_sqrd * cabs(z * conj(z));
     }
   }
-  enrg *= 4.0 * pow(M_PI, 2.0) * (0.5 / pow(Nx * Ny, 2.0));
+  /* solver.c:1487-1490: scaled per rank, then reduced */
+  for (int r = 0; r < sl->P; ++r) sl->part[r] *= 4.0 * pow(M_PI, 2.0) * (0.5 / pow(Nx * Ny, 2.0));
+  const double enrg = slab_allreduce(sl);
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
     for (long j = 0; j < Nyf; ++j) {
@@ -300,12 +361,13 @@ static int ic_decay_turb_alt(const ns2d_run_cfg* c, double complex* w, const ns2
   return 0;
 }
 
-static int ic_gauss_decay_turb(const ns2d_run_cfg* c, double complex* w, char* msg, size_t n) {
+static int ic_gauss_decay_turb(const ns2d_run_cfg* c, double complex* w, ic_slabs* sl, char* msg, size_t n) {
   const long Nx = c->N[0], Ny = c->N[1], Nyf = Ny / 2 + 1;
   double complex* psi_hat = (double complex*)malloc(sizeof(double complex) * Nx * Nyf);
   if (!psi_hat) return say(msg, n, "\n[ERROR] --- Unable to allocate memory for the [Stream Function]\n");
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
+    slab_reseed(sl, i, 0);
     for (long j = 0; j < Nyf; ++j) {
       const int k1 = (int)j;
       if (k0 == 0 && k1 == 0) {
@@ -318,20 +380,23 @@ static int ic_gauss_decay_turb(const ns2d_run_cfg* c, double complex* w, char* m
       }
     }
   }
-  double enrg = 0.0;
+  slab_clear(sl);
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
+    double* enrg = slab_part(sl, i);
     for (long j = 0; j < Nyf; ++j) {
       const int k1 = (int)j;
       if (k0 != 0 || k1 != 0) {
         const double k_sqrd = (double)(k0 * k0 + k1 * k1);
         const double complex z = psi_hat[i * Nyf + j];
-        if (j == 0 || j == Nyf - 1) enrg += k_sqrd * cabs(z * conj(z));
-        else enrg += 2.0 * k_sqrd * cabs(z * conj(z));
+        if (j == 0 || j == Nyf - 1) *enrg += k_sqrd * cabs(z * conj(z));
+        else *enrg += 2.0 * k_sqrd * cabs(z * conj(z));
       }
     }
   }
-  enrg *= 4.0 * pow(M_PI, 2.0) * (0.5 / pow(Nx * Ny, 2.0));
+  /* solver.c:1578-1581: scaled per rank, then reduced */
+  for (int r = 0; r < sl->P; ++r) sl->part[r] *= 4.0 * pow(M_PI, 2.0) * (0.5 / pow(Nx * Ny, 2.0));
+  const double enrg = slab_allreduce(sl);
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
     for (long j = 0; j < Nyf; ++j) {
@@ -347,11 +412,12 @@ static int ic_gauss_decay_turb(const ns2d_run_cfg* c, double complex* w, char* m
   return 0;
 }
 
-static void ic_extrm_ens(const ns2d_run_cfg* c, double complex* w) {
+static void ic_extrm_ens(const ns2d_run_cfg* c, double complex* w, ic_slabs* sl) {
   const long Nx = c->N[0], Nyf = c->N[1] / 2 + 1;
-  double norm = 0.0;
+  slab_clear(sl);
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
+    double* norm = slab_part(sl, i);
     for (long j = 0; j < Nyf; ++j) {
       const int k1 = (int)j;
       const double k_sqr = (double)(k0 * k0 + k1 * k1);
@@ -359,17 +425,18 @@ static void ic_extrm_ens(const ns2d_run_cfg* c, double complex* w) {
       const double wgt = annulus_weight(j, Nyf);
       if ((k_sqr > 0) && (sqrt(k_sqr) > IC_EXTRM_ENS_MIN_K)) {
         const double p = pow(sqrt(k_sqr), -IC_EXTRM_ENS_POW);
-        norm += (wgt == 1.0) ? p : 2.0 * p;
+        *norm += (wgt == 1.0) ? p : 2.0 * p;
       } else if (sqrt(k_sqr) <= IC_EXTRM_ENS_MIN_K) {
         const double p = pow(IC_EXTRM_ENS_MIN_K, -IC_EXTRM_ENS_POW);
         const double e = exp((sqrt(k_sqr) - IC_EXTRM_ENS_MIN_K) / 3.5);
-        norm += (wgt == 1.0) ? p * e : 2.0 * p * e;
+        *norm += (wgt == 1.0) ? p * e : 2.0 * p * e;
       }
     }
   }
-  norm = sqrt(IC_EXTRM_ENS_MIN_K / norm);
+  const double norm = sqrt(IC_EXTRM_ENS_MIN_K / slab_allreduce(sl));
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
+    slab_reseed(sl, i, 0);
     for (long j = 0; j < Nyf; ++j) {
       const int k1 = (int)j;
       const double k_sqr = (double)(k0 * k0 + k1 * k1);
@@ -387,22 +454,27 @@ static void ic_extrm_ens(const ns2d_run_cfg* c, double complex* w) {
   }
 }
 
-static void ic_ring(const ns2d_run_cfg* c, double complex* w) {
+static void ic_ring(const ns2d_run_cfg* c, double complex* w, ic_slabs* sl) {
   const long Nx = c->N[0], Nyf = c->N[1] / 2 + 1;
   const double mid = (IC_RING_MIN_K + IC_RING_MAX_K) / 2.0;
-  double norm = 0.0;
+  slab_clear(sl);
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
+    double* norm = slab_part(sl, i);
+    slab_reseed(sl, i, 0);
     for (long j = 0; j < Nyf; ++j) {
       const int k1 = (int)j;
       const double abs_k = sqrt((double)(k0 * k0 + k1 * k1));
       (void)urand();  /* solver.c:1690: drawn and discarded for every mode */
       if ((abs_k >= IC_RING_MIN_K && abs_k <= IC_RING_MAX_K) && (k1 != 0 || k0 > 0))
-        norm += exp(-pow(abs_k - mid, 2.0));
+        *norm += exp(-pow(abs_k - mid, 2.0));
     }
   }
+  const double norm = slab_allreduce(sl);
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
+    /* each rank's generator continues after its slab's discarded draws */
+    slab_reseed(sl, i, slab_len(sl, i, Nx) * Nyf);
     for (long j = 0; j < Nyf; ++j) {
       const int k1 = (int)j;
       const double abs_k = sqrt((double)(k0 * k0 + k1 * k1));
@@ -416,20 +488,22 @@ static void ic_ring(const ns2d_run_cfg* c, double complex* w) {
   }
 }
 
-static void ic_unif(const ns2d_run_cfg* c, double complex* w) {
+static void ic_unif(const ns2d_run_cfg* c, double complex* w, ic_slabs* sl) {
   const long Nx = c->N[0], Nyf = c->N[1] / 2 + 1;
-  double norm = 0.0;
+  slab_clear(sl);
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
+    double* norm = slab_part(sl, i);
     for (long j = 0; j < Nyf; ++j) {
       const int k1 = (int)j;
       const double k_abs = sqrt((double)(k0 * k0 + k1 * k1));
-      if (k_abs > IC_UNIF_MIN_K && k_abs < IC_UNIF_MAX_K) norm += annulus_weight(j, Nyf);
+      if (k_abs > IC_UNIF_MIN_K && k_abs < IC_UNIF_MAX_K) *norm += annulus_weight(j, Nyf);
     }
   }
-  norm = sqrt(1.0 / norm);
+  const double norm = sqrt(1.0 / slab_allreduce(sl));
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
+    slab_reseed(sl, i, 0);
     for (long j = 0; j < Nyf; ++j) {
       const int k1 = (int)j;
       const double k_abs = sqrt((double)(k0 * k0 + k1 * k1));
@@ -443,12 +517,14 @@ static void ic_unif(const ns2d_run_cfg* c, double complex* w) {
   }
 }
 
-static void ic_random(const ns2d_run_cfg* c, double complex* w) {
+static void ic_random(const ns2d_run_cfg* c, double complex* w, ic_slabs* sl) {
   const long Nx = c->N[0], Nyf = c->N[1] / 2 + 1;
   /* solver.c:1911 leaves the sum uninitialised; it starts at zero here */
-  double enst_norm = 0.0;
+  slab_clear(sl);
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
+    double* enst_norm = slab_part(sl, i);
+    slab_reseed(sl, i, 0);
     for (long j = 0; j < Nyf; ++j) {
       const int k1 = (int)j;
       const double sqrt_k = sqrt(k0 * k0 + k1 * k1);
@@ -460,10 +536,10 @@ static void ic_random(const ns2d_run_cfg* c, double complex* w) {
       } else {
         w[i * Nyf + j] = 0.0;
       }
-      enst_norm += pow(cabs(w[i * Nyf + j]), 2.0);
+      *enst_norm += pow(cabs(w[i * Nyf + j]), 2.0);
     }
   }
-  enst_norm *= 2.0;
+  const double enst_norm = slab_allreduce(sl) * 2.0;
   for (long i = 0; i < Nx; ++i) {
     const int k0 = kx_of(i, Nx);
     for (long j = 0; j < Nyf; ++j) {
@@ -491,14 +567,17 @@ static void ic_testing(const ns2d_run_cfg* c, double complex* w) {
   }
 }
 
-static void ic_none(const ns2d_run_cfg* c, double complex* w) {
+static void ic_none(const ns2d_run_cfg* c, double complex* w, ic_slabs* sl) {
   const long Nx = c->N[0], Nyf = c->N[1] / 2 + 1;
-  for (long q = 0; q < Nx * Nyf; ++q) {
-    /* solver.c:1986: amplitude drawn before the phase (gcc evaluates the
-     * left operand of the product first) */
-    const double amp = urand();
-    const double ph = urand();
-    w[q] = amp * cexp(ph * 2.0 * M_PI * I);
+  for (long i = 0; i < Nx; ++i) {
+    slab_reseed(sl, i, 0);
+    for (long j = 0; j < Nyf; ++j) {
+      /* solver.c:1986: amplitude drawn before the phase (gcc evaluates the
+       * left operand of the product first) */
+      const double amp = urand();
+      const double ph = urand();
+      w[i * Nyf + j] = amp * cexp(ph * 2.0 * M_PI * I);
+    }
   }
 }
 
@@ -528,6 +607,11 @@ static int ic_max_palin(const ns2d_run_cfg* c, double complex* w, const ns2d_tra
 int ns2d_initial_condition(const ns2d_run_cfg* c, double* w_hat, const ns2d_transforms* tf, char* msg, size_t n) {
   double complex* w = (double complex*)w_hat;
   const char* u0 = c->u0;
+  const int P = c->ic_ranks;
+  if (P < 1 || P > NS2D_IC_MAX_RANKS)
+    return say(msg, n, "\n[ERROR] --- emulated rank count [%d] must lie in 1..%d\n", P, NS2D_IC_MAX_RANKS);
+  ic_slabs sl = {P, (c->N[0] + P - 1) / P, (double*)calloc((size_t)P, sizeof(double))};
+  if (!sl.part) return say(msg, n, "\n[ERROR] --- Unable to allocate memory for the rank sums\n");
   srand(IC_SEED);
   int rc = 0;
   if (!strcmp(u0, "TG_VEL")) rc = ic_tg_vel(c, w_hat, tf, msg, n);
@@ -535,16 +619,17 @@ int ns2d_initial_condition(const ns2d_run_cfg* c, double* w_hat, const ns2d_tran
     rc = ic_real_space(c, w_hat, tf, msg, n);
   else if (!strcmp(u0, "ZERO") || !strcmp(u0, "ZERO_KOLM")) ic_zero(c, w);
   else if (!strcmp(u0, "DECAY_TURB_BB") || !strcmp(u0, "DECAY_TURB_NB") || !strcmp(u0, "DECAY_TURB_EXP"))
-    ic_decay_turb(c, w);
-  else if (!strcmp(u0, "DECAY_TURB_ALT")) rc = ic_decay_turb_alt(c, w, tf, msg, n);
-  else if (!strcmp(u0, "GAUSS_DECAY_TURB")) rc = ic_gauss_decay_turb(c, w, msg, n);
-  else if (!strcmp(u0, "EXTRM_ENS")) ic_extrm_ens(c, w);
-  else if (!strcmp(u0, "RING")) ic_ring(c, w);
-  else if (!strcmp(u0, "UNIF")) ic_unif(c, w);
+    ic_decay_turb(c, w, &sl);
+  else if (!strcmp(u0, "DECAY_TURB_ALT")) rc = ic_decay_turb_alt(c, w, tf, &sl, msg, n);
+  else if (!strcmp(u0, "GAUSS_DECAY_TURB")) rc = ic_gauss_decay_turb(c, w, &sl, msg, n);
+  else if (!strcmp(u0, "EXTRM_ENS")) ic_extrm_ens(c, w, &sl);
+  else if (!strcmp(u0, "RING")) ic_ring(c, w, &sl);
+  else if (!strcmp(u0, "UNIF")) ic_unif(c, w, &sl);
   else if (!strcmp(u0, "MAX_PALIN")) rc = ic_max_palin(c, w, tf, msg, n);
-  else if (!strcmp(u0, "RANDOM")) ic_random(c, w);
+  else if (!strcmp(u0, "RANDOM")) ic_random(c, w, &sl);
   else if (!strcmp(u0, "TESTING")) ic_testing(c, w);
-  else ic_none(c, w);
+  else ic_none(c, w, &sl);
+  free(sl.part);
   if (rc) return rc;
   ns2d_host_dealias(c, w_hat, 1);
   ns2d_host_force_conjugacy(c, w_hat);

@@ -51,6 +51,7 @@ class RunCfg(ctypes.Structure):
         ("filter", ctypes.c_char * 16),
         ("quirks", ctypes.c_uint),
         ("testing", ctypes.c_int),
+        ("ic_ranks", ctypes.c_int),
     ]
 
     def as_dict(self) -> dict:

@@ -6,7 +6,9 @@ solver command per combination of the listed grid sizes, end times, time
 steps, initial conditions, viscosities, hyperviscosity flags, CFL numbers and
 tags, with the argument layout of run_solver.py:232-245.  The commands call
 the single-GPU ns2d_solver directly instead of ``mpirun -n <procs>`` and the
-reference executable.  Bracketed lists such as ``[1.0, 2.0]`` are accepted
+reference executable; ``-R <procs>`` asks it for the initial condition that
+many MPI ranks would have drawn (the reference's random initial conditions
+depend on the rank count).  Bracketed lists such as ``[1.0, 2.0]`` are accepted
 for every list-valued key.
 
     python -m ns2d.launch -i InitFiles/TaylorGreen/tg.ini [--cmdonly]
@@ -62,10 +64,11 @@ class Job:
     file_only_mode: bool = False
     call_solver: bool = True
     num_solver_job_threads: int = 1
+    solver_procs: int = 4
 
 
 def read_ini(path: str) -> Job:
-    """The parameter parse of run_solver.py:119-214."""
+    """The parameter parse of run_solver.py:119-240."""
     if not os.path.isfile(path):
         raise FileNotFoundError(f"[ERROR] ---> File Does not exist, double check input file path: {path}")
     cp = ConfigParser()
@@ -124,13 +127,16 @@ def read_ini(path: str) -> Job:
             j.call_solver = s["call_solver"].strip().lower() in ("true", "1", "yes", "on")
         if "num_solver_job_threads" in s:
             j.num_solver_job_threads = int(s["num_solver_job_threads"])
+        if "solver_procs" in s:
+            j.solver_procs = int(s["solver_procs"])
     return j
 
 
 def solver_commands(j: Job, executable: str = SOLVER_EXE) -> List[List[str]]:
     """One argument list per parameter combination, in run_solver.py's order
     (sizes, end time, time step, initial condition, viscosity, hyperviscosity
-    flag, CFL, tag)."""
+    flag, CFL, tag), each ending in ``-R solver_procs`` (run_solver.py:270
+    runs ``mpirun -n solver_procs``)."""
     cmds = []
     for (nx, ny), t, h, u0, v, hype, c, tag in itertools.product(
             list(zip(j.Nx, j.Ny)), j.T, j.dt, j.ic, j.nu, j.hyper_visc, j.cfl, j.solver_tag):
@@ -139,7 +145,7 @@ def solver_commands(j: Job, executable: str = SOLVER_EXE) -> List[List[str]]:
             j.output_dir, nx, ny, j.t0, t, j.trans_iters, j.cfl_cond, c, h, j.step_type, v, int(hype),
             j.hypervisc_pow, j.ekmn_alpha, j.ekmn_hypo_diff, j.ekmn_hypo_pow, u0, tag, j.forcing, j.force_k,
             j.force_scale, j.save_every)
-        cmds.append([executable] + args.split())
+        cmds.append([executable] + args.split() + ["-R", str(j.solver_procs)])
     return cmds
 
 

@@ -13,8 +13,9 @@ driver's host library (libns2d_host.so), never part of the product path.
   real ``rand()`` (tests/golden/glibc_rand_123456789.txt, made by
   tests/golden/gen_glibc_rand.c).
 * ``initial_condition`` restates ``InitialConditions``
-  (Solver/src/solver.c:1107-2014) for one process holding the whole grid,
-  then ``ApplyDealiasing`` (:1039-1099) and ``ForceConjugacy`` (:2074-2107),
+  (Solver/src/solver.c:1107-2014) for one process holding the whole grid, or
+  as ``ranks`` MPI processes form it (``_Ranks``: one generator per slab of
+  kx rows, the normalisation sums reduced as ``MPI_Allreduce`` does), then ``ApplyDealiasing`` (:1039-1099) and ``ForceConjugacy`` (:2074-2107),
   as :1996-2000 do.  The scalar libm calls (``math.*``) and the operation
   order of each formula follow the C source so that spectral initial
   conditions agree bit for bit with a C build; the real-space ones go through
@@ -96,6 +97,67 @@ class GlibcRand:
         return float(self._next()) / float(RAND_MAX)
 
 
+class _Ranks:
+    """The MPI ranks of a P-process reference run as InitialConditions sees
+    them: the kx rows in FFTW's default block distribution (solver.c:2615,
+    ``FFTW_MPI_DEFAULT_BLOCK``: ceil(Nx/P) rows per rank, the last ranks
+    short or empty), one generator per rank, each seeded by the rank's own
+    ``srand(123456789)`` (solver.c:1121) and kept across the IC's loops, and
+    one partial sum per rank for each ``MPI_Allreduce``."""
+
+    def __init__(self, Nx: int, P: int):
+        if P < 1:
+            raise ValueError("rank count must be at least 1")
+        self.P = P
+        self.rows = -(-Nx // P)
+        self._gens = {}
+
+    def rng(self, i: int) -> GlibcRand:
+        """the generator of the rank that owns kx row i"""
+        r = i // self.rows
+        if r not in self._gens:
+            self._gens[r] = GlibcRand(SEED)
+        return self._gens[r]
+
+    def rank(self, i: int) -> int:
+        return i // self.rows
+
+    def sums(self) -> list:
+        return [0.0] * self.P
+
+    def allreduce(self, part: list) -> float:
+        """MPI_Allreduce(MPI_SUM) of one double per rank by recursive
+        doubling, message by message as MPICH's
+        MPIR_Allreduce_intra_recursive_doubling and Open MPI's
+        ompi_coll_base_allreduce_intra_recursivedoubling exchange them: the
+        first 2*(P - P2) ranks fold pairwise into the odd rank (P2 the
+        largest power of two <= P), then P2 ranks swap and add with the rank
+        whose new index differs in bit 0, 1, 2, ...  Returns rank 0's
+        result (every rank ends with the same bits)."""
+        P = self.P
+        buf = list(part)
+        pof2 = 1 << (P.bit_length() - 1)
+        rem = P - pof2
+        real = {}
+        for r in range(P):
+            if r < 2 * rem:
+                if r % 2 == 0:
+                    continue          # sends to r + 1 and takes no further part
+                buf[r] = buf[r - 1] + buf[r]
+                real[r // 2] = r
+            else:
+                real[r - rem] = r
+        mask = 1
+        while mask < pof2:
+            buf_next = dict((r, buf[real[nr ^ mask]] + buf[r]) for nr, r in real.items())
+            for r, v in buf_next.items():
+                buf[r] = v
+            mask <<= 1
+        out = [buf[r] for r in real.values()]
+        assert all(v == out[0] for v in out), "ranks disagree after the reduction"
+        return buf[real[0]]
+
+
 def _kx(i: int, Nx: int) -> int:
     return i if i <= Nx // 2 else i - Nx
 
@@ -154,7 +216,7 @@ def force_conjugacy(w: np.ndarray, Nx: int) -> None:
             w[i, 0] = np.conj(w[-k0, 0])
 
 
-def _decay_turb(Nx: int, Ny: int, u0: str, rng: GlibcRand) -> np.ndarray:
+def _decay_turb(Nx: int, Ny: int, u0: str, R: _Ranks) -> np.ndarray:
     """solver.c:1246-1361"""
     Nyf = Ny // 2 + 1
     w = np.zeros((Nx, Nyf), complex)
@@ -170,21 +232,22 @@ def _decay_turb(Nx: int, Ny: int, u0: str, rng: GlibcRand) -> np.ndarray:
                 spec = math.pow(sk, 6.0) / math.pow((1.0 + sk / (2.0 * DT2_K0)), 18.0)
             else:
                 spec = math.pow(sk, 7.0) / math.pow(DTEXP_K0, 8.0) * math.exp(-3.5 * math.pow(sk / DTEXP_K0, 2.0))
-            u1 = rng.uniform()
+            u1 = R.rng(i).uniform()
             if u0 == "DECAY_TURB_EXP":
                 amp = math.sqrt(sk * spec / math.pi)
             else:
                 amp = math.sqrt(spec * (sk / (2.0 * math.pi)))
             w[i, j] = _scale(amp, _expi(2.0 * math.pi * u1))
-    enrg = 0.0
+    part = R.sums()
     for i in range(Nx):
         k0 = _kx(i, Nx)
         for j in range(Nyf):
             if k0 != 0 or j != 0:
                 inv = 1.0 / float(k0 * k0 + j * j)
                 a2 = _abs2(complex(w[i, j]))
-                enrg += inv * a2 if (j == 0 or j == Nyf - 1) else 2.0 * inv * a2
-    enrg *= (0.5 / math.pow(Nx * Ny, 2.0)) * 4.0 * math.pow(math.pi, 2.0)
+                part[R.rank(i)] += inv * a2 if (j == 0 or j == Nyf - 1) else 2.0 * inv * a2
+    # solver.c:1331-1334: each rank scales its sum, then the sums are reduced
+    enrg = R.allreduce([e * ((0.5 / math.pow(Nx * Ny, 2.0)) * 4.0 * math.pow(math.pi, 2.0)) for e in part])
     e0 = {"DECAY_TURB_BB": DT_E0, "DECAY_TURB_NB": DT2_E0}.get(u0, DTEXP_E0)
     s = math.sqrt(e0 / enrg)
     for i in range(Nx):
@@ -194,15 +257,15 @@ def _decay_turb(Nx: int, Ny: int, u0: str, rng: GlibcRand) -> np.ndarray:
     return w
 
 
-def _decay_turb_alt(Nx: int, Ny: int, rng: GlibcRand, r2c, c2r) -> np.ndarray:
+def _decay_turb_alt(Nx: int, Ny: int, R: _Ranks, r2c, c2r) -> np.ndarray:
     """solver.c:1362-1522"""
     Nyf = Ny // 2 + 1
     psi_hat = np.zeros((Nx, Nyf), complex)
     for i in range(Nx):
         k0 = _kx(i, Nx)
         for j in range(Nyf):
-            u1 = rng.uniform()
-            u2 = rng.uniform()
+            u1 = R.rng(i).uniform()
+            u2 = R.rng(i).uniform()
             # cos and sin of one argument: gcc -O2 and above emits one sincos call
             e = _expi(2.0 * math.pi * u2)
             z1 = math.sqrt(-2.0 * math.log(u1)) * e.real
@@ -214,21 +277,23 @@ def _decay_turb_alt(Nx: int, Ny: int, rng: GlibcRand, r2c, c2r) -> np.ndarray:
             psi_hat[i, j] = _scale(math.sqrt(spec * math.pow(ks, 2.0) / (math.pow(ks, 3.0) * 2.0 * math.pi)),
                                    complex(z1, z2))
     psi = np.array(c2r(psi_hat), dtype=np.float64)
-    mean = 0.0
-    flat = psi.reshape(-1)
-    for q in range(flat.size):
-        flat[q] /= float(Nx * Ny)
-        mean += flat[q]
-    flat -= mean / float(Nx * Ny)
+    part = R.sums()
+    for i in range(Nx):
+        for j in range(Ny):
+            psi[i, j] /= float(Nx * Ny)
+            part[R.rank(i)] += psi[i, j]
+    mean = R.allreduce(part)
+    psi -= mean / float(Nx * Ny)
     psi_hat = np.array(r2c(psi), dtype=complex)
-    enrg = 0.0
+    part = R.sums()
     for i in range(Nx):
         k0 = _kx(i, Nx)
         for j in range(Nyf):
             k2 = float(k0 * k0 + j * j)
             a2 = _abs2(complex(psi_hat[i, j]))
-            enrg += k2 * a2 if (j == 0 or j == Nyf - 1) else 2.0 * k2 * a2
-    enrg *= 4.0 * math.pow(math.pi, 2.0) * (0.5 / math.pow(Nx * Ny, 2.0))
+            part[R.rank(i)] += k2 * a2 if (j == 0 or j == Nyf - 1) else 2.0 * k2 * a2
+    # solver.c:1487-1490
+    enrg = R.allreduce([e * (4.0 * math.pow(math.pi, 2.0) * (0.5 / math.pow(Nx * Ny, 2.0))) for e in part])
     s = math.sqrt(0.5 / enrg)
     w = np.zeros((Nx, Nyf), complex)
     for i in range(Nx):
@@ -240,7 +305,7 @@ def _decay_turb_alt(Nx: int, Ny: int, rng: GlibcRand, r2c, c2r) -> np.ndarray:
     return w
 
 
-def _gauss_decay_turb(Nx: int, Ny: int, rng: GlibcRand) -> np.ndarray:
+def _gauss_decay_turb(Nx: int, Ny: int, R: _Ranks) -> np.ndarray:
     """solver.c:1523-1606"""
     Nyf = Ny // 2 + 1
     psi = np.zeros((Nx, Nyf), complex)
@@ -251,17 +316,18 @@ def _gauss_decay_turb(Nx: int, Ny: int, rng: GlibcRand) -> np.ndarray:
                 continue
             ks = math.sqrt(float(k0 * k0 + j * j))
             spec = math.pow(ks, 6.0) / math.pow((1.0 + ks / 60.0), 18.0)
-            u1 = rng.uniform()
+            u1 = R.rng(i).uniform()
             psi[i, j] = _scale(math.sqrt(spec * (1.0 / ks)), _expi(2.0 * math.pi * u1))
-    enrg = 0.0
+    part = R.sums()
     for i in range(Nx):
         k0 = _kx(i, Nx)
         for j in range(Nyf):
             if k0 != 0 or j != 0:
                 k2 = float(k0 * k0 + j * j)
                 a2 = _abs2(complex(psi[i, j]))
-                enrg += k2 * a2 if (j == 0 or j == Nyf - 1) else 2.0 * k2 * a2
-    enrg *= 4.0 * math.pow(math.pi, 2.0) * (0.5 / math.pow(Nx * Ny, 2.0))
+                part[R.rank(i)] += k2 * a2 if (j == 0 or j == Nyf - 1) else 2.0 * k2 * a2
+    # solver.c:1578-1581
+    enrg = R.allreduce([e * (4.0 * math.pow(math.pi, 2.0) * (0.5 / math.pow(Nx * Ny, 2.0))) for e in part])
     se, sh = math.sqrt(enrg), math.sqrt(0.5)
     w = np.zeros((Nx, Nyf), complex)
     for i in range(Nx):
@@ -275,11 +341,11 @@ def _gauss_decay_turb(Nx: int, Ny: int, rng: GlibcRand) -> np.ndarray:
     return w
 
 
-def _extrm_ens(Nx: int, Ny: int, rng: GlibcRand) -> np.ndarray:
+def _extrm_ens(Nx: int, Ny: int, R: _Ranks) -> np.ndarray:
     """solver.c:1607-1670"""
     Nyf = Ny // 2 + 1
     P, K = EXTRM_ENS_POW, EXTRM_ENS_MIN_K
-    norm = 0.0
+    part = R.sums()
     for i in range(Nx):
         k0 = _kx(i, Nx)
         for j in range(Nyf):
@@ -287,79 +353,80 @@ def _extrm_ens(Nx: int, Ny: int, rng: GlibcRand) -> np.ndarray:
             edge = j == 0 or j == Nyf - 1
             if k2 > 0 and math.sqrt(k2) > K:
                 p = math.pow(math.sqrt(k2), -P)
-                norm += p if edge else 2.0 * p
+                part[R.rank(i)] += p if edge else 2.0 * p
             elif math.sqrt(k2) <= K:
                 p = math.pow(K, -P)
                 e = math.exp((math.sqrt(k2) - K) / 3.5)
-                norm += p * e if edge else 2.0 * p * e
-    norm = math.sqrt(K / norm)
+                part[R.rank(i)] += p * e if edge else 2.0 * p * e
+    norm = math.sqrt(K / R.allreduce(part))
     w = np.zeros((Nx, Nyf), complex)
     for i in range(Nx):
         k0 = _kx(i, Nx)
         for j in range(Nyf):
             k2 = float(k0 * k0 + j * j)
             if k2 > 0 and math.sqrt(k2) > K:
-                u1 = rng.uniform()
+                u1 = R.rng(i).uniform()
                 w[i, j] = _scale(math.pow(math.sqrt(k2), -P / 2.0) * norm, _expi(2.0 * math.pi * u1))
             elif math.sqrt(k2) <= K:
-                u1 = rng.uniform()
+                u1 = R.rng(i).uniform()
                 a = math.pow(K, -P / 2.0) * math.exp((math.sqrt(k2) - K) / 3.5) * norm
                 w[i, j] = _scale(a, _expi(2.0 * math.pi * u1))
     return w
 
 
-def _ring(Nx: int, Ny: int, rng: GlibcRand) -> np.ndarray:
+def _ring(Nx: int, Ny: int, R: _Ranks) -> np.ndarray:
     """solver.c:1671-1718"""
     Nyf = Ny // 2 + 1
     mid = (RING_MIN_K + RING_MAX_K) / 2.0
     inside = lambda ak, k0, j: (RING_MIN_K <= ak <= RING_MAX_K) and (j != 0 or k0 > 0)
-    norm = 0.0
+    part = R.sums()
     for i in range(Nx):
         k0 = _kx(i, Nx)
         for j in range(Nyf):
             ak = math.sqrt(float(k0 * k0 + j * j))
-            rng.uniform()
+            R.rng(i).uniform()
             if inside(ak, k0, j):
-                norm += math.exp(-math.pow(ak - mid, 2.0))
+                part[R.rank(i)] += math.exp(-math.pow(ak - mid, 2.0))
+    norm = R.allreduce(part)
     w = np.zeros((Nx, Nyf), complex)
     for i in range(Nx):
         k0 = _kx(i, Nx)
         for j in range(Nyf):
             ak = math.sqrt(float(k0 * k0 + j * j))
             if inside(ak, k0, j):
-                u1 = rng.uniform()
+                u1 = R.rng(i).uniform()
                 a = math.sqrt(72.0 * math.exp(-math.pow(ak - mid, 2.0)) / (math.pi * ak * norm))
                 w[i, j] = _scale(a, _expi(2.0 * math.pi * u1))
     return w
 
 
-def _unif(Nx: int, Ny: int, rng: GlibcRand) -> np.ndarray:
+def _unif(Nx: int, Ny: int, R: _Ranks) -> np.ndarray:
     """solver.c:1719-1765"""
     Nyf = Ny // 2 + 1
-    norm = 0.0
+    part = R.sums()
     for i in range(Nx):
         k0 = _kx(i, Nx)
         for j in range(Nyf):
             ka = math.sqrt(float(k0 * k0 + j * j))
             if UNIF_MIN_K < ka < UNIF_MAX_K:
-                norm += 1.0 if (j == 0 or j == Nyf - 1) else 2.0
-    norm = math.sqrt(1.0 / norm)
+                part[R.rank(i)] += 1.0 if (j == 0 or j == Nyf - 1) else 2.0
+    norm = math.sqrt(1.0 / R.allreduce(part))
     w = np.zeros((Nx, Nyf), complex)
     for i in range(Nx):
         k0 = _kx(i, Nx)
         for j in range(Nyf):
             ka = math.sqrt(float(k0 * k0 + j * j))
             if UNIF_MIN_K < ka < UNIF_MAX_K:
-                u1 = rng.uniform()
+                u1 = R.rng(i).uniform()
                 w[i, j] = _scale(norm, _expi(2.0 * math.pi * u1))
     return w
 
 
-def _random(Nx: int, Ny: int, rng: GlibcRand) -> np.ndarray:
+def _random(Nx: int, Ny: int, R: _Ranks) -> np.ndarray:
     """solver.c:1900-1951 (its enstrophy sum starts at zero here)"""
     Nyf = Ny // 2 + 1
     w = np.zeros((Nx, Nyf), complex)
-    enst = 0.0
+    part = R.sums()
     for i in range(Nx):
         k0 = _kx(i, Nx)
         for j in range(Nyf):
@@ -367,11 +434,11 @@ def _random(Nx: int, Ny: int, rng: GlibcRand) -> np.ndarray:
             if UNIF_MIN_K < sk < UNIF_MAX_K:
                 ek = 1.0 / sk
                 zk = math.pow(sk, 2.0) * ek
-                r1 = rng.uniform()
+                r1 = R.rng(i).uniform()
                 w[i, j] = _scale(math.sqrt(zk / ek), _expi(r1 * 2.0 * math.pi))
             z = complex(w[i, j])
-            enst += math.pow(math.hypot(z.real, z.imag), 2.0)
-    enst *= 2.0
+            part[R.rank(i)] += math.pow(math.hypot(z.real, z.imag), 2.0)
+    enst = R.allreduce(part) * 2.0
     s = math.sqrt(RAND_ENST0 / enst)
     for i in range(Nx):
         k0 = _kx(i, Nx)
@@ -395,14 +462,15 @@ def _testing(Nx: int, Ny: int) -> np.ndarray:
     return w
 
 
-def _none(Nx: int, Ny: int, rng: GlibcRand) -> np.ndarray:
+def _none(Nx: int, Ny: int, R: _Ranks) -> np.ndarray:
     """solver.c:1984-1995: amplitude drawn first, then the phase."""
     Nyf = Ny // 2 + 1
     w = np.zeros((Nx, Nyf), complex)
-    for q in range(Nx * Nyf):
-        amp = rng.uniform()
-        ph = rng.uniform()
-        w.reshape(-1)[q] = _scale(amp, _expi(ph * 2.0 * math.pi))
+    for i in range(Nx):
+        for j in range(Nyf):
+            amp = R.rng(i).uniform()
+            ph = R.rng(i).uniform()
+            w[i, j] = _scale(amp, _expi(ph * 2.0 * math.pi))
     return w
 
 
@@ -484,13 +552,14 @@ def _max_palin(Nx: int, Ny: int, psi: np.ndarray, r2c) -> np.ndarray:
 
 
 def initial_condition(u0: str, Nx: int, Ny: int, filt: str = "23", forcing: str = "NONE",
-                      r2c=None, c2r=None, psi: np.ndarray | None = None) -> np.ndarray:
-    """InitialConditions + ApplyDealiasing + ForceConjugacy (solver.c:1107-2000)."""
+                      r2c=None, c2r=None, psi: np.ndarray | None = None, ranks: int = 1) -> np.ndarray:
+    """InitialConditions + ApplyDealiasing + ForceConjugacy (solver.c:1107-2000)
+    as a run on ``ranks`` MPI processes forms it (``_Ranks``)."""
     if r2c is None:
         r2c = lambda a: np.fft.rfft2(a)
     if c2r is None:
         c2r = lambda a: np.fft.irfft2(a, s=(Nx, Ny), norm="forward")
-    rng = GlibcRand(SEED)
+    R = _Ranks(Nx, ranks)
     if u0 == "TG_VEL":
         w = _tg_vel(Nx, Ny, filt, forcing, r2c)
     elif u0 in ("TG_VORT", "DOUBLE_SHEAR_LAYER", "GAUSS_BLOB"):
@@ -498,25 +567,25 @@ def initial_condition(u0: str, Nx: int, Ny: int, filt: str = "23", forcing: str 
     elif u0 in ("ZERO", "ZERO_KOLM"):
         w = _zero(Nx, Ny, u0 == "ZERO_KOLM")
     elif u0 in ("DECAY_TURB_BB", "DECAY_TURB_NB", "DECAY_TURB_EXP"):
-        w = _decay_turb(Nx, Ny, u0, rng)
+        w = _decay_turb(Nx, Ny, u0, R)
     elif u0 == "DECAY_TURB_ALT":
-        w = _decay_turb_alt(Nx, Ny, rng, r2c, c2r)
+        w = _decay_turb_alt(Nx, Ny, R, r2c, c2r)
     elif u0 == "GAUSS_DECAY_TURB":
-        w = _gauss_decay_turb(Nx, Ny, rng)
+        w = _gauss_decay_turb(Nx, Ny, R)
     elif u0 == "EXTRM_ENS":
-        w = _extrm_ens(Nx, Ny, rng)
+        w = _extrm_ens(Nx, Ny, R)
     elif u0 == "RING":
-        w = _ring(Nx, Ny, rng)
+        w = _ring(Nx, Ny, R)
     elif u0 == "UNIF":
-        w = _unif(Nx, Ny, rng)
+        w = _unif(Nx, Ny, R)
     elif u0 == "MAX_PALIN":
         w = _max_palin(Nx, Ny, psi, r2c)
     elif u0 == "RANDOM":
-        w = _random(Nx, Ny, rng)
+        w = _random(Nx, Ny, R)
     elif u0 == "TESTING":
         w = _testing(Nx, Ny)
     else:
-        w = _none(Nx, Ny, rng)
+        w = _none(Nx, Ny, R)
     dealias(w, Nx, Ny, filt, forcing)
     force_conjugacy(w, Nx)
     return w

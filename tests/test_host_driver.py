@@ -118,10 +118,12 @@ SPECTRAL_ICS = ["DECAY_TURB_BB", "DECAY_TURB_NB", "DECAY_TURB_EXP", "GAUSS_DECAY
 REAL_ICS = ["TG_VORT", "TG_VEL", "DOUBLE_SHEAR_LAYER", "GAUSS_BLOB", "DECAY_TURB_ALT"]
 
 
-def _cfg(u0, Nx, Ny, filt="23", forcing="NONE"):
+def _cfg(u0, Nx, Ny, filt="23", forcing="NONE", ranks=None):
     args = ["-n", str(Nx), "-n", str(Ny), "-i", u0, "-D", filt]
     if forcing != "NONE":
         args += ["-f", forcing, "-f", "2", "-f", "1.0"]
+    if ranks is not None:
+        args += ["-R", str(ranks)]
     return host.parse_cli(args)
 
 
@@ -133,6 +135,118 @@ def test_initial_conditions_match_restatement(u0, Nx, Ny, filt, forcing):
     ref = R.initial_condition(u0 if u0 != "NONE" else "NONE", Nx, Ny, filt, forcing, r2c=r2c, c2r=c2r_for(Nx, Ny))
     assert got.shape == (Nx, Ny // 2 + 1)
     np.testing.assert_array_equal(got, ref)
+
+
+RANDOM_ICS = ["DECAY_TURB_BB", "DECAY_TURB_NB", "DECAY_TURB_EXP", "DECAY_TURB_ALT", "GAUSS_DECAY_TURB",
+              "EXTRM_ENS", "RING", "UNIF", "RANDOM", "NONE"]
+# (Nx, Ny, P): equal slabs, a short last slab, an empty last rank, one row per rank
+RANK_CASES = [(32, 32, 2), (32, 32, 4), (30, 16, 3), (32, 16, 6), (16, 16, 5), (24, 16, 7), (8, 16, 16)]
+
+
+@pytest.mark.parametrize("u0", RANDOM_ICS)
+@pytest.mark.parametrize("Nx,Ny,P", RANK_CASES)
+def test_rank_count_initial_conditions_match_restatement(u0, Nx, Ny, P):
+    """-R P against the restatement's P-rank form (a generator per slab kept
+    across the loops, per-slab sums reduced by recursive doubling)."""
+    got = host.initial_condition(_cfg(u0, Nx, Ny, ranks=P), r2c=r2c, c2r=c2r_for(Nx, Ny))
+    ref = R.initial_condition(u0, Nx, Ny, r2c=r2c, c2r=c2r_for(Nx, Ny), ranks=P)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_rank_count_one_is_the_default_field():
+    for u0 in ("DECAY_TURB_BB", "RING", "NONE"):
+        a = host.initial_condition(_cfg(u0, 32, 32), r2c=r2c)
+        b = host.initial_condition(_cfg(u0, 32, 32, ranks=1), r2c=r2c)
+        np.testing.assert_array_equal(a, b)
+    assert host.defaults().as_dict()["ic_ranks"] == 1
+    assert host.parse_cli(["-R", "8"]).as_dict()["ic_ranks"] == 8
+    for bad in ("0", "-2", "x", "3.5", "65537"):
+        with pytest.raises(host.HostError, match="rank count"):
+            host.parse_cli(["-R", bad])
+
+
+def _libc_uniforms(n):
+    """the first n values of (double) rand() / RAND_MAX after srand(123456789),
+    from the process's own libc"""
+    libc = ctypes.CDLL(None)
+    libc.srand(R.SEED)
+    return np.array([libc.rand() for _ in range(n)], dtype=np.float64) / float(R.RAND_MAX)
+
+
+def _hou_li_kept(Nx, Ny):
+    """modes the Hou-Li filter with integer division multiplies by exactly 1
+    (|kx| < Nx/2, ky < Ny/2) and the ky = 0 conjugacy fix-up leaves alone"""
+    return [(i, j) for i in range(Nx) for j in range(1, Ny // 2)
+            if abs(i if i <= Nx // 2 else i - Nx) < Nx // 2]
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 5])
+def test_rank_slabs_restart_the_generator(P):
+    """NONE draws an amplitude and a phase per mode (solver.c:1984-1995); on P
+    ranks each slab of ceil(Nx/P) kx rows draws the seed's sequence from its
+    start (every rank calls srand, solver.c:1121).  Checked against libc's own
+    rand(), not the restatement."""
+    Nx, Ny = 24, 16
+    Nyf, rows = Ny // 2 + 1, -(-Nx // P)
+    u = _libc_uniforms(2 * rows * Nyf)
+    w = host.initial_condition(_cfg("NONE", Nx, Ny, "HOU_LI", ranks=P), r2c=r2c)
+    for i, j in _hou_li_kept(Nx, Ny):
+        n = (i % rows) * Nyf + j               # the mode's place in its rank's walk
+        want = u[2 * n] * complex(math.cos(2.0 * math.pi * u[2 * n + 1]), math.sin(2.0 * math.pi * u[2 * n + 1]))
+        assert abs(w[i, j] - want) <= 1e-15, (P, i, j)
+
+
+@pytest.mark.parametrize("P", [1, 2, 4])
+def test_rank_ring_phases_continue_after_the_slab_discards(P):
+    """RING draws one discarded value per mode of its slab (solver.c:1690),
+    then one phase per ring mode: rank r's k-th ring mode takes draw
+    rows*Nyf + k of the seed's sequence (slabs of equal length here)."""
+    Nx, Ny = 32, 32
+    Nyf, rows = Ny // 2 + 1, Nx // P
+    w = host.initial_condition(_cfg("RING", Nx, Ny, "HOU_LI", ranks=P), r2c=r2c)
+    u = _libc_uniforms(rows * Nyf + rows * Nyf)
+    for r in range(P):
+        k = 0
+        for i in range(r * rows, (r + 1) * rows):
+            k0 = i if i <= Nx // 2 else i - Nx
+            for j in range(Nyf):
+                ak = math.sqrt(k0 * k0 + j * j)
+                if 3.0 <= ak <= 10.0 and (j != 0 or k0 > 0):
+                    ph = 2.0 * math.pi * u[rows * Nyf + k]
+                    z = w[i, j] / abs(w[i, j])
+                    assert abs(z - complex(math.cos(ph), math.sin(ph))) <= 1e-13, (P, i, j)
+                    k += 1
+        assert k > 0
+
+
+def test_rank_sums_reduce_by_recursive_doubling():
+    """Sums where the association decides the last bit: recursive doubling
+    adds pairs first ((a0+a1)+(a2+a3)), with ranks beyond the largest power of
+    two folded into pairs beforehand; a left-to-right sum gives 1.0."""
+    e = 2.0 ** -53
+    red = lambda P, v: R._Ranks(P, P).allreduce(v)
+    assert red(4, [1.0, e, e, e]) == 1.0 + 2 * e
+    assert sum([1.0, e, e, e]) == 1.0
+    assert red(3, [1.0, e, e]) == 1.0                       # (a0 + a1) + a2
+    assert red(6, [1.0, e, e, 0.0, e, e]) == 1.0 + 2 * e    # ((a0+a1) + (a2+a3)) + (a4+a5)
+    assert red(5, [1.0, e, e, e, e]) == 1.0 + 2 * e         # ((a0+a1) + a2) + (a3+a4)
+    assert red(5, [e, e, 1.0, e, e]) == 1.0 + 4 * e         # ((a0+a1) + a2) + (a3+a4): every add exact
+    assert red(3, [e, 1.0, e]) == 1.0                       # (a0+a1) + a2 rounds twice
+    assert red(1, [0.25]) == 0.25
+    # and runs whose normalisation the association does change (few cases
+    # do: a slab's sum has few terms): the C library matches the restatement
+    # and not a restatement that reduced left to right
+    for u0, Nx, Ny, P in (("DECAY_TURB_BB", 32, 16, 6), ("DECAY_TURB_BB", 24, 16, 7), ("RING", 32, 16, 6)):
+        ref = R.initial_condition(u0, Nx, Ny, ranks=P)
+        got = host.initial_condition(_cfg(u0, Nx, Ny, ranks=P), r2c=r2c)
+        np.testing.assert_array_equal(got, ref)
+        orig = R._Ranks.allreduce
+        try:
+            R._Ranks.allreduce = lambda self, part: sum(part)
+            planted = R.initial_condition(u0, Nx, Ny, ranks=P)
+        finally:
+            R._Ranks.allreduce = orig
+        assert not np.array_equal(got, planted), (u0, Nx, Ny, P)
 
 
 def test_initial_conditions_properties():
