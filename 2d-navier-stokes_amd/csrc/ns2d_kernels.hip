@@ -191,7 +191,7 @@ int rpass_variant_extra(int ny, int idx, int* W, int* PS, int* OFF, int* PK) {
   *W = v->W;
   *PS = 3;
   *OFF = 1;
-  *PK = ((v->PFR == 1 || v->PFR == 3) << 1) | ((v->PFR >= 2) << 3);
+  *PK = ((v->PFR == 1 || v->PFR == 3 || v->PFR == 4) << 1) | ((v->PFR >= 2) << 3) | ((v->PFR >= 4) << 4);
   return 0;
 }
 

@@ -564,7 +564,13 @@ constexpr int RPS = NS2D_RPASS_PS;
 // and NY/2 set to zero, as a real-input transform gives them.  Twice the
 // workgroups and three transforms per workgroup instead of five, for one
 // more forward transform per row pair.  PFR = 3: a single row whose four G
-// fields are all loaded before its first transform.
+// fields are all loaded before its first transform.  PFR = 4 and 5: PFR = 3
+// and 2 with the real product's forward transform done at half length
+// (rpass_forward_half).
+template <int NY, int TLR, bool BLOCK>
+__device__ __forceinline__ void rpass_forward_half(double2 (&w)[NY / TLR], double2* buf, const RPassArgs& a,
+                                                   int x0, int l);
+
 template <int NY, int TLR, int PAIRS, int PFR>
 __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   constexpr int T = TLR * PAIRS;
@@ -573,6 +579,8 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   constexpr int LS = padx_s<RPS>(NY) + 1;
   constexpr bool BLOCK = TLR > 64;
   constexpr bool SR = PFR >= 2;
+  constexpr bool UPF = PFR == 3 || PFR == 4;   // single row, G fields loaded up front
+  constexpr bool HLF = PFR >= 4;               // half-length forward transform
   static_assert(E >= 2 && (BLOCK ? PAIRS == 1 : 64 % TLR == 0), "bad rpass shape");
   static_assert(!SR || PAIRS == 1, "single-row shapes own one row per workgroup");
   __shared__ double2 lds[PAIRS * LS];
@@ -675,9 +683,9 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   } else if (SR && (a.mode & RP_NL) && (BLOCK || valid)) {
     double2 z1[E], z2[E];
     build(z1, 0, x0, opaque(l));
-    if constexpr (PFR == 3) build(z2, 1, x0, opaque(l));   // both field pairs loaded up front
+    if constexpr (UPF) build(z2, 1, x0, opaque(l));   // both field pairs loaded up front
     fft_reg<NY, TLR, +1, BLOCK, RPS>(z1, buf, tw, opaque(l));
-    if constexpr (PFR != 3) build(z2, 1, x0, opaque(l));
+    if constexpr (!UPF) build(z2, 1, x0, opaque(l));
     fft_reg<NY, TLR, +1, BLOCK, RPS>(z2, buf, tw, opaque(l));
 #pragma unroll
     for (int e = 0; e < E; ++e) w[e] = make_double2(z1[e].x * z2[e].x + z1[e].y * z2[e].y, 0.0);
@@ -738,13 +746,17 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   }
 
   if (SR && (a.mode & RP_NL) && (BLOCK || valid)) {
-    fft_reg<NY, TLR, -1, BLOCK, RPS>(w, buf, tw, opaque(l));
-    // k = l + TLR*e <= H: every element below E/2, lane 0's element E/2 (k = H)
+    if constexpr (HLF) {
+      rpass_forward_half<NY, TLR, BLOCK>(w, buf, a, x0, l);
+    } else {
+      fft_reg<NY, TLR, -1, BLOCK, RPS>(w, buf, tw, opaque(l));
+      // k = l + TLR*e <= H: every element below E/2, lane 0's element E/2 (k = H)
 #pragma unroll
-    for (int e = 0; e <= E / 2; ++e) {
-      const int k = l + TLR * e;
-      if (e < E / 2) a.nout[nidx(x0, k, ld)] = (e == 0 && l == 0) ? make_double2(w[e].x, 0.0) : w[e];
-      else if (l == 0) a.nout[nidx(x0, k, ld)] = make_double2(w[e].x, 0.0);
+      for (int e = 0; e <= E / 2; ++e) {
+        const int k = l + TLR * e;
+        if (e < E / 2) a.nout[nidx(x0, k, ld)] = (e == 0 && l == 0) ? make_double2(w[e].x, 0.0) : w[e];
+        else if (l == 0) a.nout[nidx(x0, k, ld)] = make_double2(w[e].x, 0.0);
+      }
     }
   } else if ((a.mode & RP_NL) && (BLOCK || valid)) {
     fft_reg<NY, TLR, -1, BLOCK, RPS>(w, buf, tw, opaque(l));
@@ -762,6 +774,71 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
         a.nout[nidx(x0, k, ld)] = make_double2(0.5 * (p.x + qm.x), 0.5 * (p.y + qm.y));
         a.nout[nidx(x0 + 1, k, ld)] = make_double2(0.5 * (p.y - qm.y), -0.5 * (p.x - qm.x));
       }
+    }
+  }
+}
+
+// The value of lane l ^ 1 (the other lane of an even/odd pair; both lanes of a
+// pair always lie in one wave): DPP quad permutation [1, 0, 3, 2] of each
+// 32-bit half.
+__device__ __forceinline__ double swap_lane_pair(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Forward transform of one real row, n(y) = w[e].x at y = l + TLR*e, as a
+// complex transform of half the length.  With z(m) = n(2m) + i n(2m+1),
+// m < M = NY/2, and Z its M-point transform, the even and odd samples'
+// transforms are E(k) = (Z(k) + conj Z(M-k))/2 and O(k) = (Z(k) - conj Z(M-k))/(2i),
+// and N(k) = E(k) + exp(-2 pi i k/NY) O(k) for k <= M (k = M: E(0) - O(0)).
+// Lanes 2j and 2j+1 hold n(2j + TLR*e) and n(2j+1 + TLR*e); each passes the
+// other half of its values across the pair, so that lane 2j holds
+// z(j + TLR*h) (its even e = 2h) and lane 2j+1 z(j + TLR/2 + TLR*h) (odd
+// e = 2h+1): the stride-TLR distribution of the M-point line over the labels
+// lh = j and j + TLR/2.  The transform leaves Z(lh + TLR*h) in the same
+// distribution; Z(M-k) comes through the line buffer.  Writes N(x0, k) for
+// k = 0..M, each once.
+template <int NY, int TLR, bool BLOCK>
+__device__ __forceinline__ void rpass_forward_half(double2 (&w)[NY / TLR], double2* buf, const RPassArgs& a,
+                                                   int x0, int l) {
+  constexpr int E = NY / TLR, EH = E / 2, M = NY / 2;
+  static_assert(E >= 4 && TLR % 2 == 0, "half-length forward transform needs E >= 4 and an even TLR");
+  const int ld = a.ld;
+  const bool odd = (l & 1) != 0;
+  const int lh = (l >> 1) + (odd ? TLR / 2 : 0);
+  double2 z[EH];
+#pragma unroll
+  for (int h = 0; h < EH; ++h) {
+    const double give = odd ? w[2 * h].x : w[2 * h + 1].x;
+    const double take = swap_lane_pair(give);
+    z[h] = odd ? make_double2(take, w[2 * h + 1].x) : make_double2(w[2 * h].x, take);
+  }
+  // exp(-2 pi i k/NY) of the outputs k = lh + TLR*h, loaded ahead of the transform
+  double2 pw[EH];
+#pragma unroll
+  for (int h = 0; h < EH; ++h) pw[h] = a.tw[(lh + TLR * h) * (NS2D_TW_LEN / NY)];
+  NS2D_TWIDDLES(M, TLR, twh, a.tw, lh)
+  fft_reg<M, TLR, -1, BLOCK, RPS>(z, buf, twh, opaque(lh));
+  // (the transform's last exchange ends with a barrier: the buffer is free)
+#pragma unroll
+  for (int h = 0; h < EH; ++h) buf[padx_s<RPS>(lh + TLR * h)] = z[h];
+  xsync<BLOCK>();
+#pragma unroll
+  for (int h = 0; h < EH; ++h) {
+    const int k = lh + TLR * h;
+    const double2 A = z[h];
+    const double2 Bm = buf[padx_s<RPS>((M - k) & (M - 1))];   // Z(M-k); conj taken below
+    const double2 Ev = make_double2(0.5 * (A.x + Bm.x), 0.5 * (A.y - Bm.y));
+    const double2 Dv = make_double2(0.5 * (A.x - Bm.x), 0.5 * (A.y + Bm.y));
+    const double2 Ov = make_double2(Dv.y, -Dv.x);
+    if (k == 0) {
+      // real-input transform: N(0) and N(M) are real
+      a.nout[nidx(x0, 0, ld)] = make_double2(Ev.x + Ov.x, 0.0);
+      a.nout[nidx(x0, M, ld)] = make_double2(Ev.x - Ov.x, 0.0);
+    } else {
+      a.nout[nidx(x0, k, ld)] = cadd(Ev, cmul(pw[h], Ov));
     }
   }
 }
@@ -1329,18 +1406,24 @@ struct KVariant {
 // points per thread), PAIRS row pairs per workgroup, W minimum waves per SIMD,
 // PFR = 1 loads both rows' G fields up front (shapes with E = 4), PFR = 2 runs
 // one row per TLR threads instead of a pair, PFR = 3 one row with its four G
-// fields loaded before the first transform; the first listed for each NY is
-// the default.  Single rows win at 256 and 1024 (these grids run in one round
-// and are bound by a workgroup's chain of transforms; at 1024 one wave per
-// row with all loads up front, no workgroup barriers: -0.6 % per step in
-// interleaved timing against 256 threads; at 512 one-wave row pairs beat
-// single rows by 1.1 %) and at 4096, where the
-// mixed-space fields' 2-row tiles make a row pair's sequential halves read
-// each 32-B run twice (single rows 260 against 287 us per launch, the two
-// rows of a tile on one XCD); at 2048 256-thread row pairs (64 against 74 us).
+// fields loaded before the first transform, PFR = 4 and 5 are PFR = 3 and 2
+// with the real product's forward transform at half length; the first listed
+// for each NY is the default.  Single rows win at every size from 256 (the
+// grids up to 1024 run in one round and are bound by a workgroup's chain of
+// transforms; at 1024 one wave per row with all loads up front, no workgroup
+// barriers: -0.6 % per step in interleaved timing against 256 threads) and at
+// 4096, where the mixed-space fields' 2-row tiles make a row pair's sequential
+// halves read each 32-B run twice (single rows 260 against 287 us per launch,
+// the two rows of a tile on one XCD).  Interleaved on one solver (median us
+// per step): 512, the half-length one-wave shape 88.6 against 96.1 for
+// one-wave row pairs and 97.3 for the same shape at full length; 2048,
+// 256-thread single rows 491.0 against 549.6 for 256-thread row pairs (491.9
+// at half length); 1024 and 4096 keep the full length (166.3 against 167.2,
+// 2302 against 2355), 256 ties (51.43 against 51.44).
 #define NS2D_RPASS_VARIANTS(X) \
   X(8, 1, 64, 1, 0) \
   X(16, 1, 64, 1, 0) \
+  X(16, 4, 1, 1, 4) \
   X(32, 2, 32, 1, 0) \
   X(32, 4, 16, 1, 0) \
   X(64, 4, 16, 1, 0) \
@@ -1354,6 +1437,8 @@ struct KVariant {
   X(256, 16, 4, 1, 0) \
   X(256, 32, 2, 1, 0) \
   X(256, 64, 1, 1, 3) \
+  X(256, 64, 1, 1, 4) \
+  X(512, 64, 1, 1, 4) \
   X(512, 64, 1, 1, 0) \
   X(512, 64, 1, 1, 2) \
   X(512, 32, 2, 1, 0) \
@@ -1375,6 +1460,10 @@ struct KVariant {
   X(1024, 256, 1, 1, 3) \
   X(1024, 64, 1, 1, 2) \
   X(1024, 128, 1, 2, 2) \
+  X(1024, 64, 1, 1, 4) \
+  X(1024, 64, 1, 1, 5) \
+  X(1024, 256, 1, 1, 5) \
+  X(2048, 256, 1, 1, 2) \
   X(2048, 256, 1, 1, 0) \
   X(2048, 128, 1, 2, 0) \
   X(2048, 128, 1, 1, 0) \
@@ -1382,6 +1471,8 @@ struct KVariant {
   X(2048, 512, 1, 1, 1) \
   X(2048, 512, 1, 1, 2) \
   X(2048, 512, 1, 1, 3) \
+  X(2048, 512, 1, 1, 5) \
+  X(2048, 256, 1, 1, 5) \
   X(4096, 512, 1, 4, 2) \
   X(4096, 512, 1, 4, 0) \
   X(4096, 256, 1, 1, 0) \
@@ -1390,7 +1481,9 @@ struct KVariant {
   X(4096, 1024, 1, 1, 0) \
   X(4096, 1024, 1, 1, 1) \
   X(4096, 1024, 1, 1, 2) \
-  X(4096, 1024, 1, 1, 3)
+  X(4096, 1024, 1, 1, 3) \
+  X(4096, 512, 1, 4, 5) \
+  X(4096, 1024, 1, 1, 5)
 
 struct RVariant {
   int NY, TLR, PAIRS, W, PFR;

@@ -187,7 +187,9 @@ int ns2d_variant_info(int pass, int n, int idx, int* shape4);
  * packed complex line (Ny/2 column slots instead of Ny/2+1); bit 1: the
  * column pass issues its RK epilogue loads before the forward transform;
  * bit 2: the packed pair's second column stays in registers; bit 3: the row
- * pass runs one row per line group instead of a row pair (row pass only). */
+ * pass runs one row per line group instead of a row pair; bit 4: that row's
+ * real product is transformed forward as a complex line of half the length
+ * (row pass only; bit 1 then means its G fields are loaded up front). */
 int ns2d_variant_extra(int pass, int n, int idx, int* extra4);
 /* The shape a new handle with this integrator (NS2D_INT_*) runs for a
  * transform length n unless NS2D_KPASS_VARIANT_<Nx> / NS2D_RPASS_VARIANT_<Ny>

@@ -47,6 +47,7 @@ void block_barrier_and_reset(unsigned t) {
 
 double emu_shfl_d[1024];
 unsigned long long emu_shfl_u[1024];
+int emu_dpp_i[1024];
 std::mutex emu_atomic_mu;
 thread_local emu_graph* emu_capturing = nullptr;
 

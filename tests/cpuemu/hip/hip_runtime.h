@@ -80,6 +80,18 @@ inline unsigned long long __shfl_xor(unsigned long long v, int off, int) {
   __builtin_amdgcn_wave_barrier();
   return r;
 }
+// DPP moves: the one permutation the kernels use, quad_perm [1, 0, 3, 2]
+// (lane l takes lane l ^ 1's value)
+extern int emu_dpp_i[1024];
+inline int __builtin_amdgcn_mov_dpp(int v, int ctrl, int, int, bool) {
+  if (ctrl != 0xB1) std::abort();
+  const unsigned t = threadIdx.x;
+  emu_dpp_i[t] = v;
+  __builtin_amdgcn_wave_barrier();
+  const int r = emu_dpp_i[t ^ 1u];
+  __builtin_amdgcn_wave_barrier();
+  return r;
+}
 extern std::mutex emu_atomic_mu;
 inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v) {
   std::lock_guard<std::mutex> lk(emu_atomic_mu);

@@ -52,7 +52,10 @@ def test_variant_tables_cover_every_size():
             assert v["threads"] == v["threads_per_pair"] * v["pairs"]
             assert v["threads_per_pair"] > 64 and v["pairs"] == 1 or 64 % v["threads_per_pair"] == 0
             assert not (v["single_row"] and v["pairs"] != 1), n
-        assert rs[0]["single_row"] == (n in (256, 1024, 4096)), n
+        assert rs[0]["single_row"] == (n >= 256), n
+        # the half-length forward transform is the default at 512 only
+        assert rs[0]["half_forward"] == (n == 512), n
+        assert all(v["single_row"] and v["per_thread"] >= 4 for v in rs if v["half_forward"]), n
         if n >= 256:
             assert any(v["single_row"] for v in rs), n
 

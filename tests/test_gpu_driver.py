@@ -64,6 +64,12 @@ def test_initial_conditions_through_the_gpu_transforms():
             got = host.initial_condition(c, solver=s)
             ref = R.initial_condition(u0, Nx, Ny)
             assert np.linalg.norm(got - ref) <= 1e-13 * np.linalg.norm(ref), u0
+        # a 4-rank run's DECAY_TURB_ALT: per-slab draws and mean, transforms on the GPU
+        c = host.parse_cli(["-n", str(Nx), "-n", str(Ny), "-i", "DECAY_TURB_ALT", "-R", "4"])
+        got = host.initial_condition(c, solver=s)
+        ref = R.initial_condition("DECAY_TURB_ALT", Nx, Ny, ranks=4)
+        assert np.linalg.norm(got - ref) <= 1e-13 * np.linalg.norm(ref)
+        assert np.linalg.norm(got - R.initial_condition("DECAY_TURB_ALT", Nx, Ny)) > 0.1 * np.linalg.norm(ref)
 
 
 def test_taylor_green_run_matches_oracle_and_layout(tmp_path):

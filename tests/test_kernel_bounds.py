@@ -190,6 +190,26 @@ def test_row_pass_indices(NY):
         assert padx_s(((NY - ky) & (NY - 1)).max()) < LS
         nyf, ld = ld_of(NY)
         assert H < ld
+        if v["half_forward"]:
+            # rpass_forward_half: lanes l and l ^ 1 swap values (one wave);
+            # lane label lh, half line of M points, outputs k = lh + TLR*h
+            assert SR and E >= 4 and TLR % 2 == 0
+            M, EH = NY // 2, E // 2
+            assert np.all((l ^ 1) // 64 == l // 64) and (l ^ 1).max() < TLR
+            lh = (l >> 1) + (l & 1) * (TLR // 2)
+            assert np.array_equal(np.sort(lh), np.arange(TLR))
+            h = np.arange(EH)[None, :]
+            m = lh[:, None] + TLR * h
+            odd = (l & 1)[:, None] == 1
+            y_even = np.where(odd, (l - 1)[:, None] + TLR * (2 * h + 1), l[:, None] + TLR * (2 * h))
+            y_odd = np.where(odd, l[:, None] + TLR * (2 * h + 1), (l + 1)[:, None] + TLR * (2 * h))
+            assert np.array_equal(y_even, 2 * m) and np.array_equal(y_odd, 2 * m + 1)
+            assert np.array_equal(np.sort(m.ravel()), np.arange(M))
+            assert fft_reg_exchanges(M, TLR) < LS
+            assert padx_s(m.max()) < LS and padx_s(((M - m) & (M - 1)).max()) < LS
+            assert (m.max()) * (8192 // NY) < 8192          # exp(-2 pi i k/NY) from the table
+            written = np.concatenate([m.ravel(), [M]])      # k = M by the k = 0 lane
+            assert np.array_equal(np.sort(written), np.arange(H + 1))
         for nx in LENGTHS:
             units = nx if SR else nx // 2
             grid = (units + PAIRS - 1) // PAIRS
