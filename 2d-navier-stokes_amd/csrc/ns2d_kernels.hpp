@@ -69,6 +69,17 @@ __device__ __forceinline__ int opaque(int v) {
   return v;
 }
 
+// Element i of a device array, addressed as the array's base plus a 32-bit
+// unsigned byte offset: the global load/store form with the base in scalar
+// registers and one 32-bit offset per lane, instead of a 64-bit address
+// formed per lane.  Every array a pass addresses this way is one field of at
+// most 4096 x 2056 complex values (135 MB; grids beyond 4096 have no shapes).
+template <typename T>
+__device__ __forceinline__ T& at(T* p, size_t i) {
+  const unsigned off = (unsigned)i * (unsigned)sizeof(T);
+  return *(T*)((const char*)p + off);
+}
+
 
 
 // ApplyDealiasing multiplier, solver.c:1039-1099
@@ -211,11 +222,11 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
         const double2 w = *pw;
         if (pre) wpre[it] = make_double2(w.x, cj ? -w.y : w.y);
         if (need_acc) {
-          const double2 av = a.acc[idx];
+          const double2 av = at(a.acc, idx);
           if (pre) apre[it] = av;
         }
         if (need_lam) {
-          const double m = a.lam[idx];
+          const double m = at(a.lam, idx);
           if (pre) lpre[it] = m;
         }
       }
@@ -227,8 +238,8 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int x = l + TL * e;
-      if (pair) R[e] = make_double2(a.nin[nidx(x, 0, ld)].x, a.nin[nidx(x, H, ld)].x);
-      else R[e] = colv ? a.nin[nidx(x, ky, ld)] : make_double2(0.0, 0.0);
+      if (pair) R[e] = make_double2(at(a.nin, nidx(x, 0, ld)).x, at(a.nin, nidx(x, H, ld)).x);
+      else R[e] = colv ? at(a.nin, nidx(x, ky, ld)) : make_double2(0.0, 0.0);
       if ((e & 3) == 3) asm volatile("" ::: "memory");
     }
     fft_reg<L, TL, -1, BLOCK, PS>(R, buf, tw, opaque(l));
@@ -246,7 +257,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
           R[e] = make_double2(0.5 * (z.x + zm.x), 0.5 * (z.y - zm.y));    // K0
           const double2 kh = make_double2(0.5 * (z.y + zm.y), -0.5 * (z.x - zm.x));  // KH
           if constexpr (PKR) RH[e] = kh;
-          else hold[L + x] = kh;
+          else at(hold, L + x) = kh;
         }
       }
       xsync<BLOCK>();
@@ -276,7 +287,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
     if (a.mode & KP_FWD) {
       if (!PKR && q == 1) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) R[e] = hold[L + lq + TL * e];
+        for (int e = 0; e < E; ++e) R[e] = at(hold, L + lq + TL * e);
       }
       const double nrm = ph.norm2;
 #pragma unroll
@@ -313,24 +324,24 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
     };
     auto acc_at = [&](int it) -> double2 {
       if constexpr (PF != 0) return apre[it];
-      return a.acc[index_of(it)];
+      return at(a.acc, index_of(it));
     };
     auto lam_at = [&](int it) -> double {
       if constexpr (PF != 0) return lpre[it];
-      return a.lam[index_of(it)];
+      return at(a.lam, index_of(it));
     };
     switch (a.epi) {
       case EPI_FROM_STATE:
 #pragma unroll
         for (int it = 0; it < E; ++it) {
-          R[it] = colv ? a.w_in[index_of(it)] : make_double2(0.0, 0.0);
+          R[it] = colv ? at(a.w_in, index_of(it)) : make_double2(0.0, 0.0);
           if ((it & 3) == 3) asm volatile("" ::: "memory");
         }
         break;
       case EPI_FROM_INPUT:
 #pragma unroll
         for (int it = 0; it < E; ++it) {
-          R[it] = colv ? a.sin[index_of(it)] : make_double2(0.0, 0.0);
+          R[it] = colv ? at(a.sin, index_of(it)) : make_double2(0.0, 0.0);
           if ((it & 3) == 3) asm volatile("" ::: "memory");
         }
         break;
@@ -338,9 +349,9 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
         if (writer && colv) {
 #pragma unroll
           for (int it = 0; it < E; ++it) {
-            a.kout[index_of(it)] = R[it];
+            at(a.kout, index_of(it)) = R[it];
             const int kx = kx_of_row(lq + TL * it, nx);
-            if (conj_fix && kx < 0) a.w_out[index_of(it)] = cconj(a.conj_src[-kx]);
+            if (conj_fix && kx < 0) at(a.w_out, index_of(it)) = cconj(at(a.conj_src, -kx));
           }
         }
         break;
@@ -350,8 +361,8 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
           for (int it = 0; it < E; ++it) {
             const double2 k = R[it];
             if (writer) {
-              a.acc[index_of(it)] = cscale(RK4_B1, k);
-              if (a.hist_store) a.hist_store[index_of(it)] = k;
+              at(a.acc, index_of(it)) = cscale(RK4_B1, k);
+              if (a.hist_store) at(a.hist_store, index_of(it)) = k;
             }
             R[it] = cadd(state_at(it), cscale(a.coef, k));     // solver.c:690
             if ((it & 3) == 3) asm volatile("" ::: "memory");
@@ -365,7 +376,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
 #pragma unroll
           for (int it = 0; it < E; ++it) {
             const double2 k = R[it];
-            if (writer) a.acc[index_of(it)] = cadd(acc_at(it), cscale(b, k));
+            if (writer) at(a.acc, index_of(it)) = cadd(acc_at(it), cscale(b, k));
             R[it] = cadd(state_at(it), cscale(a.coef, k));     // solver.c:729,768
             if ((it & 3) == 3) asm volatile("" ::: "memory");
           }
@@ -382,7 +393,7 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
             const double q = rcp_nr(2.0 + D);
             double2 nw = cadd(cscale((2.0 - D) * q, state_at(it)), cscale(2.0 * a.dt * q, sum));  // :853
             if (kx == 0 && kq == 0) nw = make_double2(0.0, 0.0);
-            if (writer) a.w_out[index_of(it)] = nw;
+            if (writer) at(a.w_out, index_of(it)) = nw;
             R[it] = nw;
             if ((it & 3) == 3) asm volatile("" ::: "memory");
           }
@@ -396,15 +407,15 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
             const size_t idx = index_of(it);
             const double2 k = R[it];
             // every mode computes; the (0,0) mode then takes 0 (solver.c:337)
-            const double2 sum = cadd(cadd(cadd(cscale(AB4_1, k), cscale(AB4_2, a.h_new[idx])),
-                                          cscale(AB4_3, a.h_mid[idx])), cscale(AB4_4, a.h_old[idx]));
-            const double D = a.dt * a.lam[idx];
+            const double2 sum = cadd(cadd(cadd(cscale(AB4_1, k), cscale(AB4_2, at(a.h_new, idx))),
+                                          cscale(AB4_3, at(a.h_mid, idx))), cscale(AB4_4, at(a.h_old, idx)));
+            const double D = a.dt * at(a.lam, idx);
             const double q = rcp_nr(2.0 + D);
             double2 nw = cadd(cscale((2.0 - D) * q, state_at(it)), cscale(2.0 * a.dt * q, sum));  // :333
             if (kx == 0 && kq == 0) nw = make_double2(0.0, 0.0);
             if (writer) {
-              a.h_store[idx] = k;   // newest history entry (solver.c:346-348)
-              a.w_out[idx] = nw;
+              at(a.h_store, idx) = k;   // newest history entry (solver.c:346-348)
+              at(a.w_out, idx) = nw;
             }
             R[it] = nw;
             if ((it & 3) == 3) asm volatile("" ::: "memory");
@@ -420,11 +431,11 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
     // ky=0 column of the next stage input, for the next ForceConjugacy
     if ((a.mode & KP_INV) && writer && kq == 0 && a.conj_dst) {
 #pragma unroll
-      for (int it = 0; it < E; ++it) a.conj_dst[lq + TL * it] = R[it];
+      for (int it = 0; it < E; ++it) at(a.conj_dst, lq + TL * it) = R[it];
     }
     if (!PKR && pair && (a.mode & KP_INV)) {
 #pragma unroll
-      for (int it = 0; it < E; ++it) hold[q * L + lq + TL * it] = R[it];
+      for (int it = 0; it < E; ++it) at(hold, q * L + lq + TL * it) = R[it];
     }
   }
 
@@ -467,8 +478,8 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
           const int x = l_o + TL * e;
           const int kx = kx_of_row(x, nx);
           // S_0 and S_H: after the column loop RH holds S_0 and R holds S_H (PK = 2)
-          const double2 s0 = PKR ? RH[PKR ? e : 0] : hold[x];
-          const double2 sh = PKR ? R[e] : hold[L + x];
+          const double2 s0 = PKR ? RH[PKR ? e : 0] : at(hold, x);
+          const double2 sh = PKR ? R[e] : at(hold, L + x);
           double2 P, Q;
           if (f == 0) {
             const double2 ps = derive(0, sh, kx, H);
@@ -504,20 +515,20 @@ __device__ __forceinline__ void kpass_body(const KPassArgs& a, const int wi) {
       for (int e = 0; e < E; ++e) {
         const size_t r0 = gidx(l_o + TL * e, 0, ld), rh = gidx(l_o + TL * e, H, ld);
         if (f == 0) {
-          gout[r0] = make_double2(0.0, 0.0);
-          gout[rh] = make_double2(0.0, F[e].x);
-          g4[2 * rh + 1] = F[e].y;
+          at(gout, r0) = make_double2(0.0, 0.0);
+          at(gout, rh) = make_double2(0.0, F[e].x);
+          at(g4, 2 * rh + 1) = F[e].y;
         } else if (f == 3) {
-          gout[r0] = make_double2(F[e].x, 0.0);
-          g4[2 * rh] = F[e].y;
+          at(gout, r0) = make_double2(F[e].x, 0.0);
+          at(g4, 2 * rh) = F[e].y;
         } else {
-          gout[r0] = make_double2(F[e].x, 0.0);
-          gout[rh] = make_double2(F[e].y, 0.0);
+          at(gout, r0) = make_double2(F[e].x, 0.0);
+          at(gout, rh) = make_double2(F[e].y, 0.0);
         }
       }
     } else if (colv) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) gout[gidx(l_o + TL * e, ky_o, ld)] = F[e];
+      for (int e = 0; e < E; ++e) at(gout, gidx(l_o + TL * e, ky_o, ld)) = F[e];
     }
   }
 }
@@ -610,18 +621,18 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
       double2 A, B;
       const double fk = (double)kk;
       if (which == 0) {         // u = i ky G1, v = G2
-        const double2 g1 = a.g[0][gidx(x, kk, ld)];
+        const double2 g1 = at(a.g[0], gidx(x, kk, ld));
         A = make_double2(-fk * g1.y, fk * g1.x);
-        B = a.g[1][gidx(x, kk, ld)];
+        B = at(a.g[1], gidx(x, kk, ld));
       } else if (which == 1) {  // w_x = G3, w_y = i ky G4
-        A = a.g[2][gidx(x, kk, ld)];
-        const double2 g4 = a.g[3][gidx(x, kk, ld)];
+        A = at(a.g[2], gidx(x, kk, ld));
+        const double2 g4 = at(a.g[3], gidx(x, kk, ld));
         B = make_double2(-fk * g4.y, fk * g4.x);
       } else if (which == 2) {  // w of rows x and x+1 = G4
-        A = a.g[3][gidx(x, kk, ld)];
-        B = a.g[3][gidx(x + 1, kk, ld)];
+        A = at(a.g[3], gidx(x, kk, ld));
+        B = at(a.g[3], gidx(x + 1, kk, ld));
       } else {                  // w of row x alone
-        A = a.g[3][gidx(x, kk, ld)];
+        A = at(a.g[3], gidx(x, kk, ld));
         B = make_double2(0.0, 0.0);
       }
       if (edge) { A.y = 0.0; B.y = 0.0; }
@@ -643,7 +654,7 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
           for (int e = 0; e < E; ++e) {
             const int ky = l + TLR * e;
             const int kk = ky > H ? NY - ky : ky;
-            pre[r][f][e] = a.g[f][gidx(x0 + r, kk, ld)];
+            pre[r][f][e] = at(a.g[f], gidx(x0 + r, kk, ld));
           }
       // the same assembly as build() from the loaded values
       auto build_pre = [&](double2* z, int which, const double2 (&g)[4][E]) {
@@ -718,8 +729,8 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
         const int y = l + TLR * e;
         const double w0 = z[e].x * a.wscale, w1 = SR ? 0.0 : z[e].y * a.wscale;
         if (a.wreal) {
-          a.wreal[(size_t)x0 * NY + y] = w0;
-          if (!SR) a.wreal[(size_t)(x0 + 1) * NY + y] = w1;
+          at(a.wreal, (size_t)x0 * NY + y) = w0;
+          if (!SR) at(a.wreal, (size_t)(x0 + 1) * NY + y) = w1;
         }
         const double aw = fmax(fabs(w0), fabs(w1));
         m = (aw > m || aw != aw) ? aw : m;
@@ -754,8 +765,8 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
 #pragma unroll
       for (int e = 0; e <= E / 2; ++e) {
         const int k = l + TLR * e;
-        if (e < E / 2) a.nout[nidx(x0, k, ld)] = (e == 0 && l == 0) ? make_double2(w[e].x, 0.0) : w[e];
-        else if (l == 0) a.nout[nidx(x0, k, ld)] = make_double2(w[e].x, 0.0);
+        if (e < E / 2) at(a.nout, nidx(x0, k, ld)) = (e == 0 && l == 0) ? make_double2(w[e].x, 0.0) : w[e];
+        else if (l == 0) at(a.nout, nidx(x0, k, ld)) = make_double2(w[e].x, 0.0);
       }
     }
   } else if ((a.mode & RP_NL) && (BLOCK || valid)) {
@@ -771,8 +782,8 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
       if (e < E / 2 || l == 0) {
         const double2 p = w[e];
         const double2 qm = cconj(buf[padx_s<RPS>((NY - k) & (NY - 1))]);
-        a.nout[nidx(x0, k, ld)] = make_double2(0.5 * (p.x + qm.x), 0.5 * (p.y + qm.y));
-        a.nout[nidx(x0 + 1, k, ld)] = make_double2(0.5 * (p.y - qm.y), -0.5 * (p.x - qm.x));
+        at(a.nout, nidx(x0, k, ld)) = make_double2(0.5 * (p.x + qm.x), 0.5 * (p.y + qm.y));
+        at(a.nout, nidx(x0 + 1, k, ld)) = make_double2(0.5 * (p.y - qm.y), -0.5 * (p.x - qm.x));
       }
     }
   }
@@ -818,7 +829,7 @@ __device__ __forceinline__ void rpass_forward_half(double2 (&w)[NY / TLR], doubl
   // exp(-2 pi i k/NY) of the outputs k = lh + TLR*h, loaded ahead of the transform
   double2 pw[EH];
 #pragma unroll
-  for (int h = 0; h < EH; ++h) pw[h] = a.tw[(lh + TLR * h) * (NS2D_TW_LEN / NY)];
+  for (int h = 0; h < EH; ++h) pw[h] = at(a.tw, (lh + TLR * h) * (NS2D_TW_LEN / NY));
   NS2D_TWIDDLES(M, TLR, twh, a.tw, lh)
   fft_reg<M, TLR, -1, BLOCK, RPS>(z, buf, twh, opaque(lh));
   // (the transform's last exchange ends with a barrier: the buffer is free)
@@ -835,10 +846,10 @@ __device__ __forceinline__ void rpass_forward_half(double2 (&w)[NY / TLR], doubl
     const double2 Ov = make_double2(Dv.y, -Dv.x);
     if (k == 0) {
       // real-input transform: N(0) and N(M) are real
-      a.nout[nidx(x0, 0, ld)] = make_double2(Ev.x + Ov.x, 0.0);
-      a.nout[nidx(x0, M, ld)] = make_double2(Ev.x - Ov.x, 0.0);
+      at(a.nout, nidx(x0, 0, ld)) = make_double2(Ev.x + Ov.x, 0.0);
+      at(a.nout, nidx(x0, M, ld)) = make_double2(Ev.x - Ov.x, 0.0);
     } else {
-      a.nout[nidx(x0, k, ld)] = cadd(Ev, cmul(pw[h], Ov));
+      at(a.nout, nidx(x0, k, ld)) = cadd(Ev, cmul(pw[h], Ov));
     }
   }
 }
