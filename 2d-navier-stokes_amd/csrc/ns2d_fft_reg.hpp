@@ -151,6 +151,67 @@ __device__ __forceinline__ void load_twiddles(TwReg<N, TL>& t, const double2* __
   load_twiddles_pass<N, TL, 1, 0>(t, tw, l);
 }
 
+// Every power w^r (r = 1..R-1) of each butterfly's twiddle, loaded from the
+// table once per kernel (w^r = exp(-2 pi i k r/(P R)) = tw[k r NS2D_TW_LEN/(P R)],
+// k r < P R): the passes then multiply by table values instead of forming the
+// powers by products (about 28 complex products per radix-16 butterfly), at
+// the cost of (R-1) E/R registers pairs per pass, for shapes whose occupancy
+// does not depend on their register count (one wave per SIMD).
+template <int N, int TL, int P>
+constexpr int tw_full_count() {
+  if constexpr (N / P <= 1) {
+    return 0;
+  } else {
+    constexpr int E = N / TL, REM = N / P;
+    constexpr int R0 = E < 16 ? E : 16;
+    constexpr int R = R0 < REM ? R0 : REM;
+    return (P > 1 ? (E / R) * (R - 1) : 0) + tw_full_count<N, TL, P * R>();
+  }
+}
+
+template <int N, int TL>
+struct TwFull {
+  static constexpr int n = tw_full_count<N, TL, 1>();
+  double2 w[n > 0 ? n : 1];
+};
+
+template <int N, int TL, int P, int OFF>
+__device__ __forceinline__ void load_twiddles_pass(TwFull<N, TL>& t, const double2* __restrict__ tw, int l) {
+  constexpr int E = N / TL, REM = N / P;
+  if constexpr (REM > 1) {
+    constexpr int R0 = E < 16 ? E : 16;
+    constexpr int R = R0 < REM ? R0 : REM;
+    constexpr int NB = E / R;
+    constexpr int TWS = NS2D_TW_LEN / (P * R);
+    if constexpr (P > 1) {
+#pragma unroll
+      for (int m = 0; m < NB; ++m) {
+        const int k = (l + TL * m) & (P - 1);
+#pragma unroll
+        for (int r = 1; r < R; ++r) t.w[OFF + m * (R - 1) + r - 1] = tw[k * r * TWS];
+      }
+    }
+    load_twiddles_pass<N, TL, P * R, OFF + (P > 1 ? NB * (R - 1) : 0)>(t, tw, l);
+  }
+}
+
+template <int N, int TL>
+__device__ __forceinline__ void load_twiddles(TwFull<N, TL>& t, const double2* __restrict__ tw, int l) {
+  load_twiddles_pass<N, TL, 1, 0>(t, tw, l);
+}
+
+// twiddle slots a butterfly of radix R uses in a set of type TW
+template <typename TW>
+struct TwSlots {
+  template <int R>
+  static constexpr int per_butterfly() { return 1; }
+};
+template <int N, int TL>
+struct TwSlots<TwFull<N, TL>> {
+  template <int R>
+  static constexpr int per_butterfly() { return R - 1; }
+};
+
 // twiddle source: the global table, or the thread's preloaded set
 template <int SIGN, int N, int TL>
 __device__ __forceinline__ double2 tw_get(const double2* __restrict__ tw, int idx, int) {
@@ -191,6 +252,21 @@ __device__ __forceinline__ int fft_slot(int p) {
   }
 }
 
+// x[r] *= w^r for butterfly m of a pass whose twiddle slots start at OFF:
+// powers formed from one twiddle (table or TwReg), or read from a TwFull set
+template <int SIGN, int N, int TL, int R, int OFF, typename TW>
+__device__ __forceinline__ void apply_butterfly_twiddles(double2* x, const TW& tw, int idx, int m) {
+  apply_twiddles<R>(x, tw_get<SIGN, N, TL>(tw, idx, OFF + m));
+}
+template <int SIGN, int N, int TL, int R, int OFF>
+__device__ __forceinline__ void apply_butterfly_twiddles(double2* x, const TwFull<N, TL>& tw, int, int m) {
+#pragma unroll
+  for (int r = 1; r < R; ++r) {
+    const double2 w = tw.w[OFF + m * (R - 1) + r - 1];
+    x[r] = cmul(x[r], SIGN < 0 ? w : make_double2(w.x, -w.y));
+  }
+}
+
 template <int N, int TL, int SIGN, bool BLOCK, int P, int PS, int OFF, typename TW>
 __device__ __forceinline__ void fft_reg_pass(double2* v, double2* buf, const TW& tw, int l) {
   constexpr int E = N / TL;
@@ -208,7 +284,7 @@ __device__ __forceinline__ void fft_reg_pass(double2* v, double2* buf, const TW&
 #pragma unroll
       for (int r = 0; r < R; ++r) x[r] = v[m + r * NB];
       const int k = i & (P - 1);
-      if constexpr (P > 1) apply_twiddles<R>(x, tw_get<SIGN, N, TL>(tw, k * TWS, OFF + m));
+      if constexpr (P > 1) apply_butterfly_twiddles<SIGN, N, TL, R, OFF>(x, tw, k * TWS, m);
       dftr<R, SIGN>(x);
       if constexpr (LAST) {
 #pragma unroll
@@ -224,7 +300,8 @@ __device__ __forceinline__ void fft_reg_pass(double2* v, double2* buf, const TW&
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = buf[fft_slot<PS, P, R>(l + TL * e)];
       xsync<BLOCK>();
-      fft_reg_pass<N, TL, SIGN, BLOCK, P * R, PS, OFF + (P > 1 ? NB : 0)>(v, buf, tw, l);
+      fft_reg_pass<N, TL, SIGN, BLOCK, P * R, PS,
+                   OFF + (P > 1 ? NB * TwSlots<TW>::template per_butterfly<R>() : 0)>(v, buf, tw, l);
     }
   }
 }

@@ -163,7 +163,7 @@ static const RVariant* find_rvariant(int ny, int idx) {
 hipError_t launch_rpass_batched(int nx, int ny, int variant, const RPassArgs* tab, int members, hipStream_t s) {
   const RVariant* v = find_rvariant(ny, variant);
   if (!v) return hipErrorInvalidValue;
-  const int units = v->PFR >= 2 ? nx : nx / 2;   // rows or row pairs
+  const int units = (v->PFR & 7) >= 2 ? nx : nx / 2;   // rows or row pairs
   const int per = (units + v->PAIRS - 1) / v->PAIRS;
   hipLaunchKernelGGL(v->batched, dim3(per * members), dim3(v->TLR * v->PAIRS), 0, s, tab, per);
   return hipGetLastError();
@@ -191,14 +191,15 @@ int rpass_variant_extra(int ny, int idx, int* W, int* PS, int* OFF, int* PK) {
   *W = v->W;
   *PS = 3;
   *OFF = 1;
-  *PK = ((v->PFR == 1 || v->PFR == 3 || v->PFR == 4) << 1) | ((v->PFR >= 2) << 3) | ((v->PFR >= 4) << 4);
+  const int pm = v->PFR & 7;
+  *PK = ((pm == 1 || pm == 3 || pm == 4) << 1) | ((pm >= 2) << 3) | ((pm >= 4) << 4) | (((v->PFR & 8) != 0) << 5);
   return 0;
 }
 
 hipError_t launch_rpass(int nx, int ny, int variant, const RPassArgs& a, hipStream_t s) {
   const RVariant* v = find_rvariant(ny, variant);
   if (!v) return hipErrorInvalidValue;
-  const int units = v->PFR >= 2 ? nx : nx / 2;   // rows or row pairs
+  const int units = (v->PFR & 7) >= 2 ? nx : nx / 2;   // rows or row pairs
   hipLaunchKernelGGL(v->kern, dim3((units + v->PAIRS - 1) / v->PAIRS), dim3(v->TLR * v->PAIRS), 0, s, a);
   return hipGetLastError();
 }

@@ -577,13 +577,19 @@ constexpr int RPS = NS2D_RPASS_PS;
 // more forward transform per row pair.  PFR = 3: a single row whose four G
 // fields are all loaded before its first transform.  PFR = 4 and 5: PFR = 3
 // and 2 with the real product's forward transform done at half length
-// (rpass_forward_half).
-template <int NY, int TLR, bool BLOCK>
+// (rpass_forward_half).  PFR + 8: the same shape with every twiddle power of
+// its transforms preloaded (TwFull) instead of formed by products.
+template <int NY, int TLR, bool BLOCK, typename TWH>
 __device__ __forceinline__ void rpass_forward_half(double2 (&w)[NY / TLR], double2* buf, const RPassArgs& a,
-                                                   int x0, int l);
+                                                   int x0, int l, const TWH& twh);
 
-template <int NY, int TLR, int PAIRS, int PFR>
-__device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
+// lane label of lane l in the half-length line of rpass_forward_half
+template <int TLR>
+__device__ __forceinline__ int half_lane(int l) { return (l >> 1) + ((l & 1) ? TLR / 2 : 0); }
+
+// tw: the row transforms' twiddle set, twh: the half-length line's
+template <int NY, int TLR, int PAIRS, int PFR, typename TW, typename TWH>
+__device__ __forceinline__ void rpass_body_tw(const RPassArgs& a, const int blk, const TW& tw, const TWH& twh) {
   constexpr int T = TLR * PAIRS;
   constexpr int H = NY / 2;
   constexpr int E = NY / TLR;
@@ -592,6 +598,7 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   constexpr bool SR = PFR >= 2;
   constexpr bool UPF = PFR == 3 || PFR == 4;   // single row, G fields loaded up front
   constexpr bool HLF = PFR >= 4;               // half-length forward transform
+  static_assert(PFR >= 0 && PFR <= 5, "rpass_body_tw takes the load mode alone");
   static_assert(E >= 2 && (BLOCK ? PAIRS == 1 : 64 % TLR == 0), "bad rpass shape");
   static_assert(!SR || PAIRS == 1, "single-row shapes own one row per workgroup");
   __shared__ double2 lds[PAIRS * LS];
@@ -603,7 +610,6 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   const bool valid = x0 < a.nx;   // uniform per block when BLOCK
   const int ld = a.ld;
   double2* buf = lds + q * LS;
-  NS2D_TWIDDLES(NY, TLR, tw, a.tw, l)
 
   // full-length spectrum of A + iB for row x (which: 0 -> u + iv, 1 -> w_x + i w_y,
   // 2 -> w(x) + i w(x+1), 3 -> w(x))
@@ -758,7 +764,7 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
 
   if (SR && (a.mode & RP_NL) && (BLOCK || valid)) {
     if constexpr (HLF) {
-      rpass_forward_half<NY, TLR, BLOCK>(w, buf, a, x0, l);
+      rpass_forward_half<NY, TLR, BLOCK>(w, buf, a, x0, l, twh);
     } else {
       fft_reg<NY, TLR, -1, BLOCK, RPS>(w, buf, tw, opaque(l));
       // k = l + TLR*e <= H: every element below E/2, lane 0's element E/2 (k = H)
@@ -789,6 +795,32 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   }
 }
 
+template <int NY, int TLR, int PAIRS, int PFR>
+__device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
+  constexpr int PM = PFR & 7;
+  const int l = (int)threadIdx.x % TLR;
+  if constexpr ((PFR & 8) != 0) {
+    static_assert(PAIRS == 1 && TLR <= 64, "preloaded twiddle powers are for one-wave rows");
+    TwFull<NY, TLR> tw;
+    load_twiddles<NY, TLR>(tw, a.tw, l);
+    if constexpr (PM >= 4) {
+      TwFull<NY / 2, TLR> twh;
+      load_twiddles<NY / 2, TLR>(twh, a.tw, half_lane<TLR>(l));
+      rpass_body_tw<NY, TLR, PAIRS, PM>(a, blk, tw, twh);
+    } else {
+      rpass_body_tw<NY, TLR, PAIRS, PM>(a, blk, tw, a.tw);
+    }
+  } else {
+    NS2D_TWIDDLES(NY, TLR, tw, a.tw, l)
+    if constexpr (PM >= 4) {
+      NS2D_TWIDDLES(NY / 2, TLR, twh, a.tw, half_lane<TLR>(l))
+      rpass_body_tw<NY, TLR, PAIRS, PM>(a, blk, tw, twh);
+    } else {
+      rpass_body_tw<NY, TLR, PAIRS, PM>(a, blk, tw, a.tw);
+    }
+  }
+}
+
 // The value of lane l ^ 1 (the other lane of an even/odd pair; both lanes of a
 // pair always lie in one wave): DPP quad permutation [1, 0, 3, 2] of each
 // 32-bit half.
@@ -811,14 +843,14 @@ __device__ __forceinline__ double swap_lane_pair(double v) {
 // lh = j and j + TLR/2.  The transform leaves Z(lh + TLR*h) in the same
 // distribution; Z(M-k) comes through the line buffer.  Writes N(x0, k) for
 // k = 0..M, each once.
-template <int NY, int TLR, bool BLOCK>
+template <int NY, int TLR, bool BLOCK, typename TWH>
 __device__ __forceinline__ void rpass_forward_half(double2 (&w)[NY / TLR], double2* buf, const RPassArgs& a,
-                                                   int x0, int l) {
+                                                   int x0, int l, const TWH& twh) {
   constexpr int E = NY / TLR, EH = E / 2, M = NY / 2;
   static_assert(E >= 4 && TLR % 2 == 0, "half-length forward transform needs E >= 4 and an even TLR");
   const int ld = a.ld;
   const bool odd = (l & 1) != 0;
-  const int lh = (l >> 1) + (odd ? TLR / 2 : 0);
+  const int lh = half_lane<TLR>(l);
   double2 z[EH];
 #pragma unroll
   for (int h = 0; h < EH; ++h) {
@@ -830,7 +862,6 @@ __device__ __forceinline__ void rpass_forward_half(double2 (&w)[NY / TLR], doubl
   double2 pw[EH];
 #pragma unroll
   for (int h = 0; h < EH; ++h) pw[h] = at(a.tw, (lh + TLR * h) * (NS2D_TW_LEN / NY));
-  NS2D_TWIDDLES(M, TLR, twh, a.tw, lh)
   fft_reg<M, TLR, -1, BLOCK, RPS>(z, buf, twh, opaque(lh));
   // (the transform's last exchange ends with a barrier: the buffer is free)
 #pragma unroll
@@ -1430,11 +1461,17 @@ struct KVariant {
 // one-wave row pairs and 97.3 for the same shape at full length; 2048,
 // 256-thread single rows 491.0 against 549.6 for 256-thread row pairs (491.9
 // at half length); 1024 and 4096 keep the full length (166.3 against 167.2,
-// 2302 against 2355), 256 ties (51.43 against 51.44).
+// 2302 against 2355), 256 ties (51.43 against 51.44).  PFR + 8 (every
+// twiddle power preloaded, one-wave rows only): 256, 48.84 against 51.06 for
+// the same shape forming the powers, so the 256 default; 512 and 1024 slower
+// (89.4 against 88.3, 185.2 against 164.7: a likely reason, not measured,
+// is the scattered table reads of every wave at the kernel's start).
 #define NS2D_RPASS_VARIANTS(X) \
   X(8, 1, 64, 1, 0) \
   X(16, 1, 64, 1, 0) \
   X(16, 4, 1, 1, 4) \
+  X(16, 4, 1, 1, 10) \
+  X(16, 4, 1, 1, 12) \
   X(32, 2, 32, 1, 0) \
   X(32, 4, 16, 1, 0) \
   X(64, 4, 16, 1, 0) \
@@ -1442,6 +1479,7 @@ struct KVariant {
   X(128, 8, 8, 1, 0) \
   X(128, 16, 4, 1, 0) \
   X(128, 32, 2, 1, 0) \
+  X(256, 64, 1, 1, 10) \
   X(256, 64, 1, 1, 2) \
   X(256, 64, 1, 1, 0) \
   X(256, 64, 1, 1, 1) \

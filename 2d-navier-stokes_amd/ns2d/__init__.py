@@ -161,7 +161,8 @@ def variants(pass_kind: int, n: int) -> list:
         x = (ctypes.c_int * 4)()
         _check(lib.ns2d_variant_extra(pass_kind, n, i, x))
         d.update(min_waves=x[0], pad_shift=x[1], line_offset=x[2], packed_pair=x[3] & 1, prefetch=(x[3] >> 1) & 1,
-                 pair_in_registers=(x[3] >> 2) & 1, single_row=(x[3] >> 3) & 1, half_forward=(x[3] >> 4) & 1)
+                 pair_in_registers=(x[3] >> 2) & 1, single_row=(x[3] >> 3) & 1, half_forward=(x[3] >> 4) & 1,
+                 twiddle_powers=(x[3] >> 5) & 1)
         out.append(d)
     return out
 

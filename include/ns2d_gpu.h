@@ -188,8 +188,10 @@ int ns2d_variant_info(int pass, int n, int idx, int* shape4);
  * column pass issues its RK epilogue loads before the forward transform;
  * bit 2: the packed pair's second column stays in registers; bit 3: the row
  * pass runs one row per line group instead of a row pair; bit 4: that row's
- * real product is transformed forward as a complex line of half the length
- * (row pass only; bit 1 then means its G fields are loaded up front). */
+ * real product is transformed forward as a complex line of half the length;
+ * bit 5: the row transforms multiply by twiddle powers preloaded from the
+ * table instead of forming them by products (row pass only; bit 1 then means
+ * its G fields are loaded up front). */
 int ns2d_variant_extra(int pass, int n, int idx, int* extra4);
 /* The shape a new handle with this integrator (NS2D_INT_*) runs for a
  * transform length n unless NS2D_KPASS_VARIANT_<Nx> / NS2D_RPASS_VARIANT_<Ny>

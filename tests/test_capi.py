@@ -53,8 +53,11 @@ def test_variant_tables_cover_every_size():
             assert v["threads_per_pair"] > 64 and v["pairs"] == 1 or 64 % v["threads_per_pair"] == 0
             assert not (v["single_row"] and v["pairs"] != 1), n
         assert rs[0]["single_row"] == (n >= 256), n
-        # the half-length forward transform is the default at 512 only
+        # the half-length forward transform is the default at 512 only, the
+        # preloaded twiddle powers at 256 only
         assert rs[0]["half_forward"] == (n == 512), n
+        assert rs[0]["twiddle_powers"] == (n == 256), n
+        assert all(v["single_row"] and v["threads_per_pair"] <= 64 for v in rs if v["twiddle_powers"]), n
         assert all(v["single_row"] and v["per_thread"] >= 4 for v in rs if v["half_forward"]), n
         if n >= 256:
             assert any(v["single_row"] for v in rs), n

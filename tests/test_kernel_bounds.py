@@ -190,6 +190,17 @@ def test_row_pass_indices(NY):
         assert padx_s(((NY - ky) & (NY - 1)).max()) < LS
         nyf, ld = ld_of(NY)
         assert H < ld
+        if v["twiddle_powers"]:
+            # TwFull: w^r = tw[k*r*NS2D_TW_LEN/(P*R)] for k < P, r < R, every pass
+            assert SR and TLR <= 64 and PAIRS == 1
+            for n in ((NY, NY // 2) if v["half_forward"] else (NY,)):
+                En, P = n // TLR, 1
+                while P < n:
+                    R = min(16, En, n // P)
+                    k = (np.arange(TLR)[:, None] + TLR * np.arange(En // R)[None, :]) & (P - 1)
+                    top = (k.max() * (R - 1)) * (8192 // (P * R))
+                    assert top < 8192, (v, n, P)
+                    P *= R
         if v["half_forward"]:
             # rpass_forward_half: lanes l and l ^ 1 swap values (one wave);
             # lane label lh, half line of M points, outputs k = lh + TLR*h
