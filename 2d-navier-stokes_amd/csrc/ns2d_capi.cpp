@@ -371,6 +371,27 @@ int state_wmax(ns2d_handle* h, double* out, double* wreal) {
   return NS2D_OK;
 }
 
+// GetTimestep's convective rate of the state: the row pass of the state's
+// fields with the CFL reduction (its N output is not used)
+int state_cfl(ns2d_handle* h, double* out) {
+  int rc = prepare_inv(h);
+  if (rc) return rc;
+  const unsigned long long zero = 0ull;
+  HIPCHK(hipMemcpyAsync(h->stat + STAT_CFL, &zero, sizeof zero, hipMemcpyHostToDevice, h->stream));
+  RPassArgs a = rbase(h, RP_NL | RP_CFL);
+  a.cmax = h->stat + STAT_CFL;
+  a.cfl_s = 1.0 / (double)((long)h->p.nx * h->p.ny);
+  a.cfl_dx = 2.0 * M_PI / (double)h->p.nx;
+  a.cfl_dy = 2.0 * M_PI / (double)h->p.ny;
+  rc = run_rpass(h, a);
+  if (rc) return rc;
+  unsigned long long st[STAT_N];
+  rc = read_stat(h, st);
+  if (rc) return rc;
+  std::memcpy(out, &st[STAT_CFL], sizeof *out);
+  return NS2D_OK;
+}
+
 // K = NonlinearRHSBatch(stage input S) into kout, S given by prepared G's
 // and conj[cur]; applies the ForceConjugacy write to the state when enabled.
 int finish_rhs_into_kout(ns2d_handle* h, bool raw) {
@@ -970,6 +991,12 @@ int ns2d_max_vort(ns2d_t* h, double* out) {
   if (!h || !out) return fail(NS2D_ERR_INVALID, "null argument");
   HIPCHK(hipSetDevice(h->p.device));
   return state_wmax(h, out, nullptr);
+}
+
+int ns2d_cfl_rate(ns2d_t* h, double* conv) {
+  if (!h || !conv) return fail(NS2D_ERR_INVALID, "null argument");
+  HIPCHK(hipSetDevice(h->p.device));
+  return state_cfl(h, conv);
 }
 
 int ns2d_real_vort(ns2d_t* h, double* w) {

@@ -73,10 +73,12 @@ enum { KP_FWD = 1, KP_INV = 2, KP_CONJFIX = 4, KP_STAT = 8, KP_ITER = 16, KP_RAW
 enum { EPI_FROM_STATE = 0, EPI_FROM_INPUT = 1, EPI_RHS = 2, EPI_RK1 = 3, EPI_RK2 = 4, EPI_RK3 = 5,
        EPI_RK4CN = 6, EPI_AB4 = 7 };
 // rpass mode bits
-enum { RP_NL = 1, RP_W = 2 };
+// RP_CFL (with RP_NL): also the maximum over the rows of pi (|u|/dx + |v|/dy)
+// into *cmax (GetTimestep's convective rate, solver.c:2425-2441)
+enum { RP_NL = 1, RP_W = 2, RP_CFL = 4 };
 
 // device status words (unsigned 64-bit)
-enum { STAT_WMAX_CUR = 0, STAT_WMAX_LAST = 1, STAT_FIRST_BAD = 2, STAT_ITER = 3, STAT_N = 4 };
+enum { STAT_WMAX_CUR = 0, STAT_WMAX_LAST = 1, STAT_FIRST_BAD = 2, STAT_ITER = 3, STAT_CFL = 4, STAT_N = 5 };
 
 // per-bin reduction quantities of the measures kernel
 enum {
@@ -138,6 +140,9 @@ struct RPassArgs {
   const double2* tw;
   int ld, mode;
   double wscale;
+  // RP_CFL: u + iv = (row transform) * cfl_s, grid spacings, result bits
+  unsigned long long* cmax;
+  double cfl_s, cfl_dx, cfl_dy;
 };
 
 // adjoint kpass modes and epilogues

@@ -603,6 +603,7 @@ __device__ __forceinline__ void rpass_body_tw(const RPassArgs& a, const int blk,
   static_assert(!SR || PAIRS == 1, "single-row shapes own one row per workgroup");
   __shared__ double2 lds[PAIRS * LS];
   __shared__ double red[(T + 63) / 64];
+  __shared__ double cred[(T + 63) / 64];
 
   const int tid = threadIdx.x;
   const int q = tid / TLR, l = tid % TLR;
@@ -647,6 +648,20 @@ __device__ __forceinline__ void rpass_body_tw(const RPassArgs& a, const int blk,
     }
   };
 
+  // RP_CFL: this thread's largest pi (|u|/dx + |v|/dy) over its points of
+  // u + iv (z1 after its transform); a NaN propagates
+  double cm = 0.0;
+  auto cfl_acc = [&](const double2* z) {
+    if (a.mode & RP_CFL) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const double uu = z[e].x * a.cfl_s, vv = z[e].y * a.cfl_s;
+        const double c = M_PI * (fabs(uu) / a.cfl_dx + fabs(vv) / a.cfl_dy);
+        cm = (c > cm || c != c) ? c : cm;
+      }
+    }
+  };
+
   double n0[E];
   double2 w[E];
   if constexpr (PFR == 1) {
@@ -687,6 +702,7 @@ __device__ __forceinline__ void rpass_body_tw(const RPassArgs& a, const int blk,
         double2 z1[E], z2[E];
         build_pre(z1, 0, pre[r]);
         fft_reg<NY, TLR, +1, BLOCK, RPS>(z1, buf, tw, opaque(l));
+        cfl_acc(z1);
         build_pre(z2, 1, pre[r]);
         fft_reg<NY, TLR, +1, BLOCK, RPS>(z2, buf, tw, opaque(l));
 #pragma unroll
@@ -702,6 +718,7 @@ __device__ __forceinline__ void rpass_body_tw(const RPassArgs& a, const int blk,
     build(z1, 0, x0, opaque(l));
     if constexpr (UPF) build(z2, 1, x0, opaque(l));   // both field pairs loaded up front
     fft_reg<NY, TLR, +1, BLOCK, RPS>(z1, buf, tw, opaque(l));
+    cfl_acc(z1);
     if constexpr (!UPF) build(z2, 1, x0, opaque(l));
     fft_reg<NY, TLR, +1, BLOCK, RPS>(z2, buf, tw, opaque(l));
 #pragma unroll
@@ -713,6 +730,7 @@ __device__ __forceinline__ void rpass_body_tw(const RPassArgs& a, const int blk,
       double2 z1[E], z2[E];
       build(z1, 0, x, opaque(l));
       fft_reg<NY, TLR, +1, BLOCK, RPS>(z1, buf, tw, opaque(l));
+      cfl_acc(z1);
       build(z2, 1, x, opaque(l));
       fft_reg<NY, TLR, +1, BLOCK, RPS>(z2, buf, tw, opaque(l));
 #pragma unroll
@@ -721,6 +739,26 @@ __device__ __forceinline__ void rpass_body_tw(const RPassArgs& a, const int blk,
         if (r == 0) n0[e] = nl;
         else w[e] = make_double2(n0[e], nl);
       }
+    }
+  }
+
+  if (a.mode & RP_CFL) {
+    // wave then block max, as for max|w| below (its own LDS words)
+    unsigned long long mb = (unsigned long long)__double_as_longlong(cm != cm ? __longlong_as_double(0x7ff8000000000000ll) : cm);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(mb, off, 64);
+      mb = o > mb ? o : mb;
+    }
+    if ((tid & 63) == 0) cred[tid >> 6] = __longlong_as_double((long long)mb);
+    __syncthreads();
+    if (tid == 0) {
+      unsigned long long bm = 0;
+      for (int i = 0; i < (T + 63) / 64; ++i) {
+        const unsigned long long v = (unsigned long long)__double_as_longlong(cred[i]);
+        bm = v > bm ? v : bm;
+      }
+      atomicMax(a.cmax, bm);
     }
   }
 

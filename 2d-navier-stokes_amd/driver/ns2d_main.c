@@ -153,42 +153,14 @@ static void print_update(run* r, long iters, double t, double dt, double T, long
 }
 
 /* GetTimestep (solver.c:2376-2462).  The CFL branch uses the velocity of
- * every mode (the reference's row offset at solver.c:2394 drops it). */
+ * every mode (the reference's row offset at solver.c:2394 drops it); its
+ * convective rate max pi (|u|/dx + |v|/dy) is reduced on the GPU. */
 static double next_dt(run* r, double dt, double w_max_init) {
   const ns2d_run_cfg* c = &r->c;
   if (c->cfl_cond_flag == 1) {
-    gpu_check(ns2d_get_state(r->h, r->w_hat), "state download");
-    double complex* uh = (double complex*)alloc_d(2 * r->Nx * r->Nyf);
-    double complex* vh = (double complex*)alloc_d(2 * r->Nx * r->Nyf);
-    const double complex* w = (const double complex*)r->w_hat;
-    for (long i = 0; i < r->Nx; ++i) {
-      const int k0 = (int)(i <= r->Nx / 2 ? i : i - r->Nx);
-      for (long j = 0; j < r->Nyf; ++j) {
-        const int k1 = (int)j;
-        const long q = i * r->Nyf + j;
-        if (k0 != 0 || k1 != 0) {
-          const double complex iok = I / (double)(k0 * k0 + k1 * k1);
-          uh[q] = ((double)k1) * iok * w[q];
-          vh[q] = -1.0 * ((double)k0) * iok * w[q];
-        } else {
-          uh[q] = vh[q] = 0.0;
-        }
-      }
-    }
-    double* u = alloc_d(r->Nx * r->Ny);
-    double* v = alloc_d(r->Nx * r->Ny);
-    gpu_check(ns2d_c2r(r->h, (const double*)uh, u), "velocity transform");
-    gpu_check(ns2d_c2r(r->h, (const double*)vh, v), "velocity transform");
     const double dx = 2.0 * M_PI / (double)r->Nx, dy = 2.0 * M_PI / (double)r->Ny;
     double conv = 0.0;
-    for (long q = 0; q < r->Nx * r->Ny; ++q) {
-      const double uu = u[q] * (1.0 / (double)(r->Nx * r->Ny)), vv = v[q] * (1.0 / (double)(r->Nx * r->Ny));
-      conv = fmax(M_PI * (fabs(uu) / dx + fabs(vv) / dy), conv);
-    }
-    free(uh);
-    free(vh);
-    free(u);
-    free(v);
+    gpu_check(ns2d_cfl_rate(r->h, &conv), "CFL rate");
     const double diff = pow(M_PI, 2.0) * (c->nu / pow(dx, 2.0) + c->nu / pow(dy, 2.0));
     return c->cfl_const / (conv + diff);
   }

@@ -62,7 +62,7 @@ class _Info(ctypes.Structure):
 
 EXPORTS = ["ns2d_params_default", "ns2d_create", "ns2d_destroy", "ns2d_set_state", "ns2d_get_state",
            "ns2d_set_state_device", "ns2d_get_state_device", "ns2d_step", "ns2d_rhs", "ns2d_measures",
-           "ns2d_max_vort", "ns2d_real_vort", "ns2d_r2c", "ns2d_c2r", "ns2d_get_info", "ns2d_sync",
+           "ns2d_max_vort", "ns2d_cfl_rate", "ns2d_real_vort", "ns2d_r2c", "ns2d_c2r", "ns2d_get_info", "ns2d_sync",
            "ns2d_get_stream", "ns2d_profile_steps", "ns2d_variant_count", "ns2d_variant_info", "ns2d_variant_extra",
            "ns2d_default_variant", "ns2d_set_variant", "ns2d_ens_create", "ns2d_ens_member", "ns2d_ens_step", "ns2d_ens_autotune", "ns2d_ens_variants", "ns2d_ens_sync",
            "ns2d_ens_destroy", "ns2d_vjp", "ns2d_vjp_device", "ns2d_measures_noalloc", "ns2d_layout", "ns2d_last_error"]
@@ -111,6 +111,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ns2d_measures": ([vp, i, P(_Measures), dp, dp, dp], i),
         "ns2d_measures_noalloc": ([vp, i, P(_Measures), dp, dp, dp], i),
         "ns2d_max_vort": ([vp, dp], i),
+        "ns2d_cfl_rate": ([vp, dp], i),
         "ns2d_real_vort": ([vp, dp], i),
         "ns2d_r2c": ([vp, dp, dp], i),
         "ns2d_c2r": ([vp, dp, dp], i),
@@ -318,6 +319,13 @@ class Solver:
     def max_vort(self) -> float:
         v = ctypes.c_double()
         _check(_lib.ns2d_max_vort(self._h, ctypes.byref(v)))
+        return v.value
+
+    def cfl_rate(self) -> float:
+        """GetTimestep's convective rate max pi (|u|/dx + |v|/dy) of the
+        state (solver.c:2381-2441), reduced on the device."""
+        v = ctypes.c_double()
+        _check(_lib.ns2d_cfl_rate(self._h, ctypes.byref(v)))
         return v.value
 
     def real_vorticity(self) -> np.ndarray:

@@ -142,6 +142,15 @@ int ns2d_measures_noalloc(ns2d_t* h, int with_flux, ns2d_measures_t* m, double* 
 /* GetMaxData("VORT") (solver.c:2171-2295): max |c2r(w_hat)| / (Nx*Ny). */
 int ns2d_max_vort(ns2d_t* h, double* out);
 
+/* GetTimestep's convective rate for the CFL step (solver.c:2381-2441): the
+ * maximum over the grid of pi (|u|/dx + |v|/dy) of the current state, u and v
+ * from u_hat = i ky/|k|^2 w_hat, v_hat = -i kx/|k|^2 w_hat, c2r / (Nx*Ny),
+ * dx = 2 pi/Nx, dy = 2 pi/Ny (every mode's own velocity: the reference's row
+ * offset at solver.c:2394 is not reproduced).  The caller forms
+ * dt = cfl / (conv + pi^2 nu (1/dx^2 + 1/dy^2)) as solver.c:2443-2447 does.
+ * Reduced on the device: no host copy of the fields. */
+int ns2d_cfl_rate(ns2d_t* h, double* conv);
+
 /* The real-space vorticity written as "w" (hdf5_funcs.c:174,796):
  * w = c2r(w_hat)/(Nx*Ny), host [Nx][Ny] doubles. */
 int ns2d_real_vort(ns2d_t* h, double* w);

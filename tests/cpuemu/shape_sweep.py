@@ -4,6 +4,7 @@ past the last column and past the row's end (ky >= ld), with the per-mode
 Hou-Li filter table, a non-Hermitian ky = 0 column and data in the Nyquist
 column: RK4CN steps (direct and graph-replayed), CN-AB4 steps, the RHS, the
 measures with flux terms (and the allocation-free variant), r2c, c2r, the
+CFL rate,
 adjoint, and an ensemble whose tuner launches every batched shape.  Any
 out-of-bounds access aborts the process with an AddressSanitizer report.
 
@@ -50,6 +51,7 @@ def exercise(s, w, x):
     s.r2c(x)
     s.c2r(w)
     s.max_vort()
+    s.cfl_rate()
 
 
 def main():

@@ -244,10 +244,15 @@ def test_pass_variants_match_oracle(Nx, Ny, var, rvar, integ):
         s.spectral_step(1e-3, 6)
         got = s.get_state()
         mv = s.max_vort()
+        cr = s.cfl_rate()
         K = s.nonlinear_rhs(w0)
         wr = s.c2r(w0)
     assert rel(got, st.w_hat) < 1e-12
     assert rel(K, O.nonlinear_rhs(w0, p)) < 1e-12
+    # GetTimestep's convective rate, reduced in the row pass (RP_CFL): with
+    # nu = 0 and cfl = 1 the oracle's CFL step is 1 / rate
+    cr_ref = 1.0 / O.cfl_timestep(st.w_hat, O.Params(Nx, Ny, nu=0.0), 1.0)
+    assert abs(cr - cr_ref) <= 1e-12 * cr_ref
     # the row pass's vorticity transform (max|w| check, c2r)
     assert abs(mv - O.max_vort(st.w_hat, p)) <= 1e-12 * abs(mv)
     assert rel(wr, O.c2r(w0, Nx, Ny)) < 1e-13
