@@ -200,7 +200,8 @@ hipError_t launch_rpass(int nx, int ny, int variant, const RPassArgs& a, hipStre
   const RVariant* v = find_rvariant(ny, variant);
   if (!v) return hipErrorInvalidValue;
   const int units = (v->PFR & 7) >= 2 ? nx : nx / 2;   // rows or row pairs
-  hipLaunchKernelGGL(v->kern, dim3((units + v->PAIRS - 1) / v->PAIRS), dim3(v->TLR * v->PAIRS), 0, s, a);
+  hipLaunchKernelGGL((a.mode & RP_CFL) ? v->cfl : v->kern, dim3((units + v->PAIRS - 1) / v->PAIRS),
+                     dim3(v->TLR * v->PAIRS), 0, s, a);
   return hipGetLastError();
 }
 

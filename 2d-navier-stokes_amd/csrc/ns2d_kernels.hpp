@@ -587,8 +587,10 @@ __device__ __forceinline__ void rpass_forward_half(double2 (&w)[NY / TLR], doubl
 template <int TLR>
 __device__ __forceinline__ int half_lane(int l) { return (l >> 1) + ((l & 1) ? TLR / 2 : 0); }
 
-// tw: the row transforms' twiddle set, twh: the half-length line's
-template <int NY, int TLR, int PAIRS, int PFR, typename TW, typename TWH>
+// tw: the row transforms' twiddle set, twh: the half-length line's.  CFL:
+// the instantiation that can reduce the CFL rate (mode RP_CFL); the step's
+// kernels are built without that code.
+template <int NY, int TLR, int PAIRS, int PFR, bool CFL, typename TW, typename TWH>
 __device__ __forceinline__ void rpass_body_tw(const RPassArgs& a, const int blk, const TW& tw, const TWH& twh) {
   constexpr int T = TLR * PAIRS;
   constexpr int H = NY / 2;
@@ -603,7 +605,7 @@ __device__ __forceinline__ void rpass_body_tw(const RPassArgs& a, const int blk,
   static_assert(!SR || PAIRS == 1, "single-row shapes own one row per workgroup");
   __shared__ double2 lds[PAIRS * LS];
   __shared__ double red[(T + 63) / 64];
-  __shared__ double cred[(T + 63) / 64];
+  __shared__ double cred[CFL ? (T + 63) / 64 : 1];
 
   const int tid = threadIdx.x;
   const int q = tid / TLR, l = tid % TLR;
@@ -652,7 +654,7 @@ __device__ __forceinline__ void rpass_body_tw(const RPassArgs& a, const int blk,
   // u + iv (z1 after its transform); a NaN propagates
   double cm = 0.0;
   auto cfl_acc = [&](const double2* z) {
-    if (a.mode & RP_CFL) {
+    if (CFL && (a.mode & RP_CFL)) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const double uu = z[e].x * a.cfl_s, vv = z[e].y * a.cfl_s;
@@ -742,7 +744,7 @@ __device__ __forceinline__ void rpass_body_tw(const RPassArgs& a, const int blk,
     }
   }
 
-  if (a.mode & RP_CFL) {
+  if (CFL && (a.mode & RP_CFL)) {
     // wave then block max, as for max|w| below (its own LDS words)
     unsigned long long mb = (unsigned long long)__double_as_longlong(cm != cm ? __longlong_as_double(0x7ff8000000000000ll) : cm);
 #pragma unroll
@@ -833,7 +835,7 @@ __device__ __forceinline__ void rpass_body_tw(const RPassArgs& a, const int blk,
   }
 }
 
-template <int NY, int TLR, int PAIRS, int PFR>
+template <int NY, int TLR, int PAIRS, int PFR, bool CFL = false>
 __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
   constexpr int PM = PFR & 7;
   const int l = (int)threadIdx.x % TLR;
@@ -844,17 +846,17 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a, const int blk) {
     if constexpr (PM >= 4) {
       TwFull<NY / 2, TLR> twh;
       load_twiddles<NY / 2, TLR>(twh, a.tw, half_lane<TLR>(l));
-      rpass_body_tw<NY, TLR, PAIRS, PM>(a, blk, tw, twh);
+      rpass_body_tw<NY, TLR, PAIRS, PM, CFL>(a, blk, tw, twh);
     } else {
-      rpass_body_tw<NY, TLR, PAIRS, PM>(a, blk, tw, a.tw);
+      rpass_body_tw<NY, TLR, PAIRS, PM, CFL>(a, blk, tw, a.tw);
     }
   } else {
     NS2D_TWIDDLES(NY, TLR, tw, a.tw, l)
     if constexpr (PM >= 4) {
       NS2D_TWIDDLES(NY / 2, TLR, twh, a.tw, half_lane<TLR>(l))
-      rpass_body_tw<NY, TLR, PAIRS, PM>(a, blk, tw, twh);
+      rpass_body_tw<NY, TLR, PAIRS, PM, CFL>(a, blk, tw, twh);
     } else {
-      rpass_body_tw<NY, TLR, PAIRS, PM>(a, blk, tw, a.tw);
+      rpass_body_tw<NY, TLR, PAIRS, PM, CFL>(a, blk, tw, a.tw);
     }
   }
 }
@@ -929,6 +931,12 @@ __device__ __forceinline__ void rpass_forward_half(double2 (&w)[NY / TLR], doubl
 template <int NY, int TLR, int PAIRS, int W, int PFR>
 __global__ __launch_bounds__(TLR * PAIRS, W) void rpass_kernel(RPassArgs a) {
   rpass_body<NY, TLR, PAIRS, PFR>(a, xcd_remap((int)blockIdx.x, (int)gridDim.x));
+}
+
+// the same shape with the CFL-rate reduction compiled in (ns2d_cfl_rate)
+template <int NY, int TLR, int PAIRS, int W, int PFR>
+__global__ __launch_bounds__(TLR * PAIRS, W) void rpass_cfl_kernel(RPassArgs a) {
+  rpass_body<NY, TLR, PAIRS, PFR, true>(a, xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
 template <int NY, int TLR, int PAIRS, int W, int PFR>
@@ -1576,6 +1584,7 @@ struct RVariant {
   int NY, TLR, PAIRS, W, PFR;
   void (*kern)(RPassArgs);
   void (*batched)(const RPassArgs*, int);
+  void (*cfl)(RPassArgs);   // kern with the CFL-rate reduction (mode RP_CFL)
 };
 
 // The shape tables, assembled from the per-length translation units in

@@ -32,7 +32,8 @@ struct RAdd {
 template <int NY, int TLR, int PAIRS, int W, int PFR>
 struct RAdd<true, NY, TLR, PAIRS, W, PFR> {
   static void add(std::vector<RVariant>& v) {
-    v.push_back({NY, TLR, PAIRS, W, PFR, rpass_kernel<NY, TLR, PAIRS, W, PFR>, rpass_batched<NY, TLR, PAIRS, W, PFR>});
+    v.push_back({NY, TLR, PAIRS, W, PFR, rpass_kernel<NY, TLR, PAIRS, W, PFR>, rpass_batched<NY, TLR, PAIRS, W, PFR>,
+                 rpass_cfl_kernel<NY, TLR, PAIRS, W, PFR>});
   }
 };
 
