@@ -84,9 +84,9 @@ hipError_t launch_kpass_batched(int nx, int nyf, int variant, const KPassArgs* t
 }
 
 // Default column-pass shape: index 0 of its length for both integrators (at
-// 4096 the unpacked 512-thread shape with the 16-slot padding wins for the
-// RK4CN stages, 327 against 332 us per launch for the packed 1024-thread one,
-// and for CN-AB4's heavier epilogue, 455 against 457 us).
+// 4096 the unpacked 1024-thread shape with the 16-slot padding: 320.5
+// against 326.9 us per RK4CN launch for the 512-thread one, 444.2 against
+// 445.0 us for CN-AB4's heavier epilogue, in alternating bench runs).
 int kpass_default_variant(int nx, bool ab4) {
   (void)nx;
   (void)ab4;

@@ -1465,6 +1465,7 @@ __global__ __launch_bounds__(T) void measures_kernel(MeasArgs a) {
   X(2048, 2, 512, 1, 3, 1, 2, 1, 0) \
   X(2048, 2, 512, 1, 3, 1, 2, 4, 0) \
   X(2048, 2, 512, 1, 3, 8, 0, 1, 0) \
+  X(4096, 2, 1024, 1, 4, 8, 0, 1, 0) \
   X(4096, 2, 512, 1, 4, 8, 0, 1, 0) \
   X(4096, 2, 1024, 1, 3, 4, 1, 1, 0) \
   X(4096, 2, 512, 1, 3, 1, 1, 1, 0) \

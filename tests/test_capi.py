@@ -39,7 +39,7 @@ def test_variant_tables_cover_every_size():
         assert vs[0]["packed_pair"] == 0, n
         if n == 4096:
             assert any(v["packed_pair"] for v in vs)
-            assert vs[0]["threads"] == 512 and vs[0]["cols"] == 2 and vs[0]["pad_shift"] == 4
+            assert vs[0]["threads"] == 1024 and vs[0]["cols"] == 2 and vs[0]["pad_shift"] == 4
         assert all(v["min_waves"] >= 1 and v["pad_shift"] >= 2 for v in vs), n
         # defaults: index 0 of each length for both integrators
         for integ in ("RK4CN", "AB4"):
